@@ -1,0 +1,901 @@
+// Host side of the C-ABI declared in include/tvbk.h: plan construction
+// (validation, derived constants, connectome upload, buffer layout, kernel
+// shape), state upload/export, step launches and monitor read-back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "tvbk.h"
+#include "tvbk_kernels.hpp"
+
+using namespace tvbk;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct ModelInfo {
+  int nmp, nsv, ncv, ndp;
+};
+
+bool model_info(int model, ModelInfo* mi) {
+  switch (model) {
+    case TVBK_MODEL_G2D: *mi = {12, 2, 1, 13}; return true;
+    case TVBK_MODEL_MPR: *mi = {8, 2, 2, 8}; return true;
+    case TVBK_MODEL_JR: *mi = {15, 6, 2, 14}; return true;
+    case TVBK_MODEL_SUM: *mi = {0, 1, 1, 0}; return true;
+  }
+  return false;
+}
+
+int cfun_nparam(int cf) {
+  switch (cf) {
+    case TVBK_CFUN_LINEAR: return 2;
+    case TVBK_CFUN_SIGMOID_JR: return 5;
+    case TVBK_CFUN_IDENTITY: return 0;
+  }
+  return -1;
+}
+
+bool combo_supported(int model, int cf) {
+  if (cf == TVBK_CFUN_LINEAR) return true;
+  if (cf == TVBK_CFUN_SIGMOID_JR) return model == TVBK_MODEL_JR;   // pre needs two cvars
+  if (cf == TVBK_CFUN_IDENTITY) return model == TVBK_MODEL_SUM;
+  return false;
+}
+
+int n_channel(int model, int cf) {
+  ModelInfo mi;
+  model_info(model, &mi);
+  return cf == TVBK_CFUN_SIGMOID_JR ? 1 : mi.ncv;
+}
+int n_term(int model, int cf) { return n_channel(model, cf); }
+
+// Derived constants the kernels use, evaluated in float64 in the order the
+// reference evaluates them (see the model structs in tvbk_device.hpp).
+void derive_model(int model, const double* p, double* out) {
+  switch (model) {
+    case TVBK_MODEL_G2D: {  // tau I a b c d e f g alpha beta gamma
+      const double tau = p[0], I = p[1], a = p[2], b = p[3], c = p[4], d = p[5], e = p[6], f = p[7], g = p[8],
+                   alpha = p[9], beta = p[10], gamma = p[11];
+      const double o[13] = {d * tau, alpha, f, e, g, gamma * I, gamma, a, b, c, d, beta, tau};
+      std::memcpy(out, o, sizeof(o));
+      break;
+    }
+    case TVBK_MODEL_MPR: {  // tau I Delta J eta Gamma cr cv
+      const double tau = p[0], I = p[1], Delta = p[2], J = p[3], eta = p[4], cr = p[6], cv = p[7];
+      const double pi = 3.141592653589793;
+      const double o[8] = {1 / tau, Delta / (pi * tau), (pi * pi) * (tau * tau), eta, J * tau, I, cr, cv};
+      std::memcpy(out, o, sizeof(o));
+      break;
+    }
+    case TVBK_MODEL_JR: {  // A B a b v0 nu_max r J a_1 a_2 a_3 a_4 p_min p_max mu
+      const double A = p[0], B = p[1], a = p[2], b = p[3], v0 = p[4], nu = p[5], r = p[6], J = p[7], a1 = p[8],
+                   a2 = p[9], a3 = p[10], a4 = p[11], mu = p[14];
+      const double o[14] = {2.0 * nu, r,       v0,     a1 * J, a3 * J, A * a,   2.0 * a,
+                            a * a,    mu,      a2 * J, B * b,  a4 * J, 2.0 * b, b * b};
+      std::memcpy(out, o, sizeof(o));
+      break;
+    }
+    default: break;
+  }
+}
+
+void derive_cfun(int cf, const double* p, double* out) {
+  if (cf == TVBK_CFUN_LINEAR) {
+    out[0] = p[0];
+    out[1] = p[1];
+  } else if (cf == TVBK_CFUN_SIGMOID_JR) {  // cmin cmax midpoint r a
+    out[0] = p[0];
+    out[1] = p[1] - p[0];
+    out[2] = p[3];
+    out[3] = p[2];
+    out[4] = p[4];
+  }
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct tvbk_plan {
+  tvbk_plan_desc d;
+  ModelInfo mi;
+  int nch = 0, nct = 0, ncp = 0, prow = 0;
+  int kernel = 0, T = 1, waves = 1, nthreads = 64;
+  int esize = 4;
+  bool stochastic = false;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int max_n = 0;
+  int64_t max_nnz = 0;
+  int spatial = 0;
+  // tiles
+  std::vector<TileDesc> tiles;
+  std::vector<int> tile_of, lane_of;  // per instance
+  std::vector<int64_t> cur_step;      // per instance
+  std::vector<int64_t> remaining;     // per instance (-1 = unlimited)
+  std::vector<ConnDesc> conns;
+  // LDS layout
+  int lds_bold = 0, lds_state = 0, lds_c = 0, lds_edge = 0, lds_w = 0, lds_rp = 0, lds_total = 0;
+  int64_t img_bytes = 0;
+  bool csr_in_lds = false;
+  int mon_lds_off[TVBK_MAX_MON] = {0};
+  int mon_Q[TVBK_MAX_MON] = {0};
+  // device buffers
+  DevBuf d_tiles, d_conns, d_edges, d_w, d_rp, d_ring, d_img, d_bacc, d_params, d_flags, d_streams, d_z;
+  DevBuf d_taps[TVBK_MAX_MON], d_out[TVBK_MAX_MON];
+  int64_t z_steps = 0;
+  const double* z_ptr = nullptr;
+  // last run bookkeeping
+  int64_t last_n = 0;
+  std::vector<int64_t> mon_count[TVBK_MAX_MON], mon_first[TVBK_MAX_MON], mon_offset[TVBK_MAX_MON];
+  int64_t mon_bytes[TVBK_MAX_MON] = {0};
+  int64_t out_cap_samples[TVBK_MAX_MON] = {0};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool have_timing = false;
+  std::vector<double> taps_host[TVBK_MAX_MON];
+};
+
+namespace {
+
+int fail(tvbk_plan* p, int code, const std::string& msg) {
+  if (p) p->err = msg; else g_err = msg;
+  return code;
+}
+
+#define HIPCHK(plan, expr)                                                                  \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess)                                                                   \
+      return fail(plan, TVBK_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
+  } while (0)
+
+int dev_alloc(tvbk_plan* p, DevBuf* b, size_t bytes) {
+  b->bytes = bytes;
+  if (bytes == 0) { b->p = nullptr; return TVBK_OK; }
+  hipError_t e = hipMalloc(&b->p, bytes);
+  if (e != hipSuccess) {
+    b->p = nullptr;
+    return fail(p, TVBK_ENOMEM, "hipMalloc(" + std::to_string(bytes) + " bytes) failed: " + hipGetErrorString(e));
+  }
+  e = hipMemsetAsync(b->p, 0, bytes, p->stream);
+  if (e != hipSuccess) return fail(p, TVBK_EHIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+  return TVBK_OK;
+}
+
+void dev_free(DevBuf* b) {
+  if (b->p) hipFree(b->p);
+  b->p = nullptr;
+  b->bytes = 0;
+}
+
+int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+// Emission steps of a monitor with period istep within [s0, s0+n).
+void emissions(int64_t s0, int64_t n, int istep, int64_t* count, int64_t* first) {
+  if (n <= 0) { *count = 0; *first = s0; return; }
+  const int64_t f = ((s0 + istep - 1) / istep) * istep;
+  const int64_t last = s0 + n - 1;
+  *first = f;
+  *count = (f > last) ? 0 : (last - f) / istep + 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tvbk_abi_version(void) { return TVBK_ABI_VERSION; }
+
+int tvbk_model_info(int32_t model, int32_t cfun, int32_t* n_model_param, int32_t* n_cfun_param, int32_t* n_svar,
+                    int32_t* n_ch) {
+  ModelInfo mi;
+  if (!model_info(model, &mi)) return fail(nullptr, TVBK_EINVAL, "unknown model");
+  const int ncp = cfun_nparam(cfun);
+  if (ncp < 0) return fail(nullptr, TVBK_EINVAL, "unknown coupling function");
+  if (!combo_supported(model, cfun)) return fail(nullptr, TVBK_EUNSUPPORTED, "model/coupling combination not supported");
+  if (n_model_param) *n_model_param = mi.nmp;
+  if (n_cfun_param) *n_cfun_param = ncp;
+  if (n_svar) *n_svar = mi.nsv;
+  if (n_ch) *n_ch = n_channel(model, cfun);
+  return TVBK_OK;
+}
+
+const char* tvbk_last_error(tvbk_plan* plan) { return plan ? plan->err.c_str() : g_err.c_str(); }
+
+static void destroy(tvbk_plan* p) {
+  DevBuf* bufs[] = {&p->d_tiles, &p->d_conns, &p->d_edges, &p->d_w,      &p->d_rp,     &p->d_ring,
+                    &p->d_img,   &p->d_bacc,  &p->d_params, &p->d_flags, &p->d_streams, &p->d_z};
+  for (DevBuf* b : bufs) dev_free(b);
+  for (int m = 0; m < TVBK_MAX_MON; ++m) {
+    dev_free(&p->d_taps[m]);
+    dev_free(&p->d_out[m]);
+  }
+  if (p->ev0) hipEventDestroy(p->ev0);
+  if (p->ev1) hipEventDestroy(p->ev1);
+  delete p;
+}
+
+int tvbk_plan_destroy(tvbk_plan* plan) {
+  if (!plan) return TVBK_OK;
+  hipStreamSynchronize(plan->stream);
+  destroy(plan);
+  return TVBK_OK;
+}
+
+static int layout_lds(tvbk_plan* p, int T, int64_t max_lds, bool allow_csr) {
+  const tvbk_plan_desc& d = p->d;
+  const int N = p->max_n;
+  int64_t off = 0;
+  p->lds_bold = 0;
+  for (int m = 0; m < d.n_mon; ++m)
+    if (d.mon[m].kind == TVBK_MON_BOLD) {
+      p->mon_lds_off[m] = (int)off;
+      off += (int64_t)N * d.mon[m].n_voi * T * 8;
+    }
+  p->lds_state = (int)off;
+  off += (int64_t)N * p->mi.nsv * T * p->esize;
+  for (int m = 0; m < d.n_mon; ++m)
+    if (d.mon[m].kind == TVBK_MON_TAVG) {
+      off = align_up(off, 8);
+      p->mon_lds_off[m] = (int)off;
+      off += (int64_t)N * d.mon[m].n_voi * 2 * T * p->esize;
+    }
+  off = align_up(off, 16);
+  p->img_bytes = off;
+  p->lds_c = (int)off;
+  off += 2 * (int64_t)N * p->nct * T * p->esize;
+  off = align_up(off, 16);
+  const int64_t csr = align_up(p->max_nnz * 4, 16) + align_up(p->max_nnz * p->esize, 16) + align_up((N + 1) * 4, 16);
+  p->csr_in_lds = allow_csr && (off + csr <= max_lds);
+  if (p->csr_in_lds) {
+    p->lds_edge = (int)off;
+    off += align_up(p->max_nnz * 4, 16);
+    p->lds_w = (int)off;
+    off += align_up(p->max_nnz * p->esize, 16);
+    p->lds_rp = (int)off;
+    off += align_up((N + 1) * 4, 16);
+  }
+  p->lds_total = (int)off;
+  return off <= max_lds ? TVBK_OK : TVBK_EUNSUPPORTED;
+}
+
+int tvbk_plan_create(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out) {
+  if (!desc || !out) return fail(nullptr, TVBK_EINVAL, "null argument");
+  *out = nullptr;
+  if (desc->abi_version != TVBK_ABI_VERSION) return fail(nullptr, TVBK_EINVAL, "ABI version mismatch");
+  tvbk_plan* p = new tvbk_plan();
+  p->d = *desc;
+  p->stream = (hipStream_t)stream;
+  const tvbk_plan_desc& d = p->d;
+  auto bail = [&](int code) {
+    g_err = p->err;
+    destroy(p);
+    return code;
+  };
+  if (!model_info(d.model, &p->mi)) { p->err = "unknown model"; return bail(TVBK_EINVAL); }
+  p->ncp = cfun_nparam(d.cfun);
+  if (p->ncp < 0) { p->err = "unknown coupling function"; return bail(TVBK_EINVAL); }
+  if (!combo_supported(d.model, d.cfun)) {
+    p->err = "unsupported simulator component combination (model/coupling)";
+    return bail(TVBK_EUNSUPPORTED);
+  }
+  if (d.scheme < TVBK_EULER_DET || d.scheme > TVBK_IDENTITY) { p->err = "unknown integrator"; return bail(TVBK_EINVAL); }
+  p->stochastic = d.scheme == TVBK_EULER_STO || d.scheme == TVBK_HEUN_STO;
+  if (p->stochastic && d.noise == TVBK_NOISE_NONE) { p->err = "stochastic scheme needs a noise source"; return bail(TVBK_EINVAL); }
+  if (!p->stochastic && d.noise != TVBK_NOISE_NONE) { p->err = "deterministic scheme takes no noise"; return bail(TVBK_EINVAL); }
+  if (d.precision != TVBK_F32 && d.precision != TVBK_F64) { p->err = "bad precision"; return bail(TVBK_EINVAL); }
+  p->esize = d.precision == TVBK_F64 ? 8 : 4;
+  if (d.n_inst <= 0 || d.n_conn <= 0 || !d.conns || !d.params) { p->err = "empty plan"; return bail(TVBK_EINVAL); }
+  const int nparam_expect = p->mi.nmp + p->ncp + (p->stochastic ? p->mi.nsv : 0);
+  if (d.n_param != nparam_expect) {
+    p->err = "n_param " + std::to_string(d.n_param) + " != " + std::to_string(nparam_expect);
+    return bail(TVBK_EINVAL);
+  }
+  p->nch = n_channel(d.model, d.cfun);
+  p->nct = n_term(d.model, d.cfun);
+  p->prow = p->mi.ndp + p->ncp + p->mi.nsv;
+  if (d.n_obs <= 0 || d.n_obs > TVBK_MAX_OBS || !d.obs_coef) { p->err = "bad observables"; return bail(TVBK_EINVAL); }
+  if (d.n_mon < 0 || d.n_mon > TVBK_MAX_MON) { p->err = "bad monitor count"; return bail(TVBK_EINVAL); }
+  if (d.dt <= 0) { p->err = "dt must be positive"; return bail(TVBK_EINVAL); }
+  if (d.max_steps <= 0) { p->err = "max_steps must be positive"; return bail(TVBK_EINVAL); }
+
+  // ---- connectomes
+  std::vector<uint32_t> edges;
+  std::vector<double> w64;
+  std::vector<int32_t> rp;
+  for (int c = 0; c < d.n_conn; ++c) {
+    const tvbk_connectome& cn = d.conns[c];
+    if (cn.n_node <= 0 || cn.n_node > 65535 || cn.horizon <= 0 || cn.horizon > 65536 || cn.nnz < 0) {
+      p->err = "connectome " + std::to_string(c) + ": n_node must be in [1, 65535] and horizon in [1, 65536]";
+      return bail(TVBK_EINVAL);
+    }
+    ConnDesc cd{};
+    cd.n_node = cn.n_node;
+    cd.horizon = cn.horizon;
+    cd.nnz = (int32_t)cn.nnz;
+    cd.rp_off = (int64_t)rp.size();
+    cd.e_off = (int64_t)edges.size();
+    cd.d_min = cn.horizon;
+    if (cn.row_ptr[0] != 0 || cn.row_ptr[cn.n_node] != cn.nnz) { p->err = "row_ptr inconsistent with nnz"; return bail(TVBK_EINVAL); }
+    for (int i = 0; i < cn.n_node; ++i) {
+      const int n = cn.row_ptr[i + 1] - cn.row_ptr[i];
+      if (n < 0 || n > 1025) { p->err = "CSR rows must have 0..1025 entries"; return bail(TVBK_EINVAL); }
+    }
+    for (int i = 0; i <= cn.n_node; ++i) rp.push_back(cn.row_ptr[i]);
+    for (int64_t e = 0; e < cn.nnz; ++e) {
+      const int j = cn.col[e], dl = cn.idelay[e];
+      if (j < 0 || j >= cn.n_node || dl < 0 || dl >= cn.horizon || dl > 65535) {
+        p->err = "edge " + std::to_string(e) + " of connectome " + std::to_string(c) + " out of range";
+        return bail(TVBK_EINVAL);
+      }
+      cd.d_min = std::min(cd.d_min, dl);
+      edges.push_back((uint32_t)j | ((uint32_t)dl << 16));
+      w64.push_back(cn.weight[e]);
+    }
+    p->max_n = std::max(p->max_n, cn.n_node);
+    p->max_nnz = std::max<int64_t>(p->max_nnz, cn.nnz);
+    p->conns.push_back(cd);
+  }
+  std::vector<int> inst_conn(d.n_inst, 0);
+  bool shared = true;
+  for (int b = 0; b < d.n_inst; ++b) {
+    inst_conn[b] = d.inst_conn ? d.inst_conn[b] : 0;
+    if (inst_conn[b] < 0 || inst_conn[b] >= d.n_conn) { p->err = "inst_conn out of range"; return bail(TVBK_EINVAL); }
+    if (inst_conn[b] != inst_conn[0]) shared = false;
+  }
+  for (int m = 0; m < d.n_mon; ++m) {
+    const tvbk_monitor& mo = d.mon[m];
+    if (mo.kind < TVBK_MON_RAW || mo.kind > TVBK_MON_BOLD || mo.istep <= 0 || mo.n_voi <= 0 || mo.n_voi > TVBK_MAX_OBS) {
+      p->err = "bad monitor " + std::to_string(m);
+      return bail(TVBK_EINVAL);
+    }
+    for (int v = 0; v < mo.n_voi; ++v)
+      if (mo.voi[v] < 0 || mo.voi[v] >= d.n_obs) { p->err = "monitor voi out of range"; return bail(TVBK_EINVAL); }
+    if (mo.kind == TVBK_MON_BOLD) {
+      if (mo.interim_istep <= 0 || mo.n_taps <= 0 || !mo.taps) { p->err = "bad Bold monitor"; return bail(TVBK_EINVAL); }
+      // live accumulators: emissions within S interim samples of a sample
+      p->mon_Q[m] = (int)(((int64_t)mo.n_taps * mo.interim_istep - 1) / mo.istep + 2);
+      p->taps_host[m].assign(mo.taps, mo.taps + mo.n_taps);
+    }
+  }
+
+  // ---- kernel shape
+  int dev = 0;
+  HIPCHK(p, hipGetDevice(&dev));
+  int max_lds = 65536;
+  hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  int kernel = d.kernel;
+  if (kernel == TVBK_KERNEL_AUTO) kernel = (shared && d.n_inst >= 32) ? TVBK_KERNEL_BATCH_LANE : TVBK_KERNEL_EDGE_LANE;
+  if (kernel == TVBK_KERNEL_DENSE_MFMA) { p->err = "dense MFMA kernel not available in this build"; return bail(TVBK_EUNSUPPORTED); }
+  if (kernel == TVBK_KERNEL_BATCH_LANE && !shared) { p->err = "batch-lane kernel needs one shared connectome"; return bail(TVBK_EINVAL); }
+  p->kernel = kernel;
+  int T = 1;
+  if (kernel == TVBK_KERNEL_BATCH_LANE) {
+    if (d.tile) {
+      T = d.tile;
+    } else {
+      T = 16;
+      for (int cand : {64, 32}) {
+        if ((d.n_inst + cand - 1) / cand >= 256) { T = cand; break; }
+      }
+    }
+    if (T != 16 && T != 32 && T != 64) { p->err = "tile must be 16, 32 or 64"; return bail(TVBK_EINVAL); }
+    while (layout_lds(p, T, max_lds, true) != TVBK_OK) {
+      if (d.tile || T == 16) { p->err = "tile does not fit in LDS"; return bail(TVBK_EUNSUPPORTED); }
+      T /= 2;
+    }
+  } else {
+    if (d.tile && d.tile != 1) { p->err = "edge-lane kernel uses tile 1"; return bail(TVBK_EINVAL); }
+    if (layout_lds(p, 1, max_lds, true) != TVBK_OK) { p->err = "node state does not fit in LDS"; return bail(TVBK_EUNSUPPORTED); }
+  }
+  p->T = T;
+  {
+    const int P = p->max_n * std::max(p->nch, 1);
+    int nw;
+    if (d.waves) nw = d.waves;
+    else if (T > 1) nw = std::min(16, std::max(1, (P + 64 / T - 1) / (64 / T)));
+    else nw = std::min(16, std::max(1, (std::max(P, p->max_n) + 63) / 64));
+    if (nw < 1 || nw > 16) { p->err = "waves must be in [1, 16]"; return bail(TVBK_EINVAL); }
+    p->waves = nw;
+    p->nthreads = 64 * nw;
+  }
+
+  // ---- tiles
+  const int B = d.n_inst;
+  p->tile_of.assign(B, 0);
+  p->lane_of.assign(B, 0);
+  p->cur_step.assign(B, 0);
+  p->remaining.assign(B, -1);
+  std::vector<int> order(B);
+  std::iota(order.begin(), order.end(), 0);
+  if (T == 1) {
+    // largest connectomes first: in-order dispatch then fills CUs
+    // longest-processing-time first
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+      const ConnDesc& cx = p->conns[inst_conn[x]];
+      const ConnDesc& cy = p->conns[inst_conn[y]];
+      return (int64_t)cx.nnz + cx.n_node > (int64_t)cy.nnz + cy.n_node;
+    });
+  }
+  int64_t ring_off = 0, img_off = 0, bacc_off = 0;
+  const int ntiles = (B + T - 1) / T;
+  p->tiles.resize(ntiles);
+  for (int k = 0; k < ntiles; ++k) {
+    TileDesc& td = p->tiles[k];
+    std::memset(&td, 0, sizeof(td));
+    const int b0 = (T == 1) ? order[k] : k * T;
+    td.inst0 = b0;
+    td.n_inst = (T == 1) ? 1 : std::min(T, B - b0);
+    td.conn = inst_conn[b0];
+    const ConnDesc& cd = p->conns[td.conn];
+    for (int l = 0; l < td.n_inst; ++l) {
+      p->tile_of[b0 + l] = k;
+      p->lane_of[b0 + l] = l;
+    }
+    td.ring_off = ring_off;
+    ring_off += (int64_t)cd.horizon * cd.n_node * p->nch * T;
+    if ((int64_t)cd.horizon * cd.n_node * p->nch * T >= (1LL << 31)) { p->err = "tile history exceeds 2^31 elements"; return bail(TVBK_EUNSUPPORTED); }
+    td.img_off = img_off;
+    img_off += align_up(p->img_bytes, 256);
+    for (int m = 0; m < d.n_mon; ++m)
+      if (d.mon[m].kind == TVBK_MON_BOLD) {
+        td.bacc_off[m] = bacc_off;
+        bacc_off += (int64_t)p->mon_Q[m] * d.mon[m].n_voi * cd.n_node * T;
+      }
+  }
+
+  // ---- derived parameters
+  const bool spatial = d.node_param_mask != 0;
+  if (spatial && (d.n_conn != 1 || !d.node_params)) { p->err = "node parameters need one connectome and node_params"; return bail(TVBK_EINVAL); }
+  p->spatial = spatial ? 1 : 0;
+  const int Nsp = spatial ? p->conns[0].n_node : 1;
+  std::vector<double> prm((size_t)B * Nsp * p->prow, 0.0);
+  std::vector<double> raw(d.n_param);
+  for (int b = 0; b < B; ++b)
+    for (int i = 0; i < Nsp; ++i) {
+      for (int k = 0; k < d.n_param; ++k) {
+        raw[k] = d.params[(size_t)b * d.n_param + k];
+        if (spatial && ((d.node_param_mask >> k) & 1ull)) raw[k] = d.node_params[(size_t)k * Nsp + i];
+      }
+      double* o = &prm[((size_t)b * Nsp + i) * p->prow];
+      derive_model(d.model, raw.data(), o);
+      derive_cfun(d.cfun, raw.data() + p->mi.nmp, o + p->mi.ndp);
+      for (int v = 0; v < p->mi.nsv; ++v)
+        o[p->mi.ndp + p->ncp + v] = p->stochastic ? std::sqrt(2.0 * raw[p->mi.nmp + p->ncp + v]) : 0.0;
+    }
+
+  // ---- allocate and upload
+  int rc;
+  auto up = [&](DevBuf* b, const void* src, size_t bytes) -> int {
+    int r = dev_alloc(p, b, bytes);
+    if (r) return r;
+    if (bytes) {
+      hipError_t e = hipMemcpy(b->p, src, bytes, hipMemcpyHostToDevice);
+      if (e != hipSuccess) return fail(p, TVBK_EHIP, std::string("upload: ") + hipGetErrorString(e));
+    }
+    return TVBK_OK;
+  };
+  if ((rc = up(&p->d_conns, p->conns.data(), p->conns.size() * sizeof(ConnDesc)))) return bail(rc);
+  if ((rc = up(&p->d_edges, edges.data(), edges.size() * 4 + 16))) return bail(rc);
+  if ((rc = up(&p->d_rp, rp.data(), rp.size() * 4))) return bail(rc);
+  if (p->esize == 8) {
+    if ((rc = up(&p->d_w, w64.data(), w64.size() * 8 + 16))) return bail(rc);
+    if ((rc = up(&p->d_params, prm.data(), prm.size() * 8))) return bail(rc);
+  } else {
+    std::vector<float> w32(w64.begin(), w64.end()), p32(prm.begin(), prm.end());
+    if ((rc = up(&p->d_w, w32.data(), w32.size() * 4 + 16))) return bail(rc);
+    if ((rc = up(&p->d_params, p32.data(), p32.size() * 4))) return bail(rc);
+  }
+  if (d.inst_stream) {
+    if ((rc = up(&p->d_streams, d.inst_stream, (size_t)B * 8))) return bail(rc);
+  }
+  for (int m = 0; m < d.n_mon; ++m)
+    if (d.mon[m].kind == TVBK_MON_BOLD)
+      if ((rc = up(&p->d_taps[m], p->taps_host[m].data(), p->taps_host[m].size() * 8))) return bail(rc);
+  if ((rc = dev_alloc(p, &p->d_ring, (size_t)ring_off * p->esize))) return bail(rc);
+  if ((rc = dev_alloc(p, &p->d_img, (size_t)img_off))) return bail(rc);
+  if ((rc = dev_alloc(p, &p->d_bacc, (size_t)bacc_off * 8))) return bail(rc);
+  if ((rc = dev_alloc(p, &p->d_flags, (size_t)B * 4))) return bail(rc);
+  if ((rc = dev_alloc(p, &p->d_tiles, p->tiles.size() * sizeof(TileDesc)))) return bail(rc);
+  // monitor outputs sized for max_steps per call
+  for (int m = 0; m < d.n_mon; ++m) {
+    const int64_t cap = d.max_steps / d.mon[m].istep + 1;
+    p->out_cap_samples[m] = cap;
+    const int es = d.mon[m].kind == TVBK_MON_BOLD ? 8 : p->esize;
+    int64_t elems = 0;
+    if (T > 1) {
+      elems = cap * d.mon[m].n_voi * (int64_t)p->conns[0].n_node * B;
+    } else {
+      for (int b = 0; b < B; ++b) elems += cap * d.mon[m].n_voi * (int64_t)p->conns[inst_conn[b]].n_node;
+    }
+    if ((rc = dev_alloc(p, &p->d_out[m], (size_t)elems * es))) return bail(rc);
+    p->mon_count[m].assign(B, 0);
+    p->mon_first[m].assign(B, 0);
+    p->mon_offset[m].assign(B, 0);
+  }
+  HIPCHK(p, hipEventCreate(&p->ev0));
+  HIPCHK(p, hipEventCreate(&p->ev1));
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  *out = p;
+  return TVBK_OK;
+}
+
+int tvbk_plan_shape(tvbk_plan* p, int32_t* kernel, int32_t* tile, int32_t* waves, int32_t* lds_bytes) {
+  if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
+  if (kernel) *kernel = p->kernel;
+  if (tile) *tile = p->T;
+  if (waves) *waves = p->waves;
+  if (lds_bytes) *lds_bytes = p->lds_total;
+  return TVBK_OK;
+}
+
+static int conn_of_inst(tvbk_plan* p, int b) { return p->tiles[p->tile_of[b]].conn; }
+
+int tvbk_set_history(tvbk_plan* p, int32_t inst, const void* history, int32_t elem_bytes, const double* state,
+                     int64_t current_step) {
+  if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
+  if (elem_bytes != 4 && elem_bytes != 8) return fail(p, TVBK_EINVAL, "elem_bytes must be 4 or 8");
+  if (current_step < 0) return fail(p, TVBK_EINVAL, "current_step must be >= 0");
+  const int B = p->d.n_inst;
+  if (inst < -1 || inst >= B) return fail(p, TVBK_EINVAL, "instance out of range");
+  const int b0 = inst < 0 ? 0 : inst, b1 = inst < 0 ? B : inst + 1;
+  const int conn = conn_of_inst(p, b0);
+  if (inst < 0)
+    for (int b = 0; b < B; ++b)
+      if (conn_of_inst(p, b) != conn) return fail(p, TVBK_EINVAL, "inst=-1 needs one shared connectome");
+  const ConnDesc& cd = p->conns[conn];
+  const int H = cd.horizon, N = cd.n_node, nch = p->nch, nsv = p->mi.nsv;
+  const size_t nh = (size_t)H * nch * N;
+  // stage compact arrays in plan precision on the device
+  std::vector<unsigned char> hbuf(nh * p->esize), sbuf((size_t)nsv * N * p->esize);
+  std::vector<unsigned char> hin(nh * elem_bytes);
+  HIPCHK(p, hipMemcpy(hin.data(), history, hin.size(), hipMemcpyDefault));
+  std::vector<double> sin((size_t)nsv * N);
+  HIPCHK(p, hipMemcpy(sin.data(), state, sin.size() * 8, hipMemcpyDefault));
+  for (size_t k = 0; k < nh; ++k) {
+    const double v = elem_bytes == 8 ? reinterpret_cast<double*>(hin.data())[k] : reinterpret_cast<float*>(hin.data())[k];
+    if (p->esize == 8) reinterpret_cast<double*>(hbuf.data())[k] = v;
+    else reinterpret_cast<float*>(hbuf.data())[k] = (float)v;
+  }
+  for (size_t k = 0; k < sin.size(); ++k) {
+    if (p->esize == 8) reinterpret_cast<double*>(sbuf.data())[k] = sin[k];
+    else reinterpret_cast<float*>(sbuf.data())[k] = (float)sin[k];
+  }
+  void *dh = nullptr, *ds = nullptr;
+  HIPCHK(p, hipMalloc(&dh, hbuf.size()));
+  HIPCHK(p, hipMalloc(&ds, sbuf.size()));
+  hipMemcpy(dh, hbuf.data(), hbuf.size(), hipMemcpyHostToDevice);
+  hipMemcpy(ds, sbuf.data(), sbuf.size(), hipMemcpyHostToDevice);
+  hipError_t e = hipSuccess;
+  for (int b = b0; b < b1 && e == hipSuccess; ++b) {
+    const TileDesc& td = p->tiles[p->tile_of[b]];
+    e = launch_scatter(p->d.precision, false, (char*)p->d_ring.p + td.ring_off * p->esize,
+                       (unsigned char*)p->d_img.p + td.img_off + p->lds_state, dh, ds, H, N, nch, nsv, p->T,
+                       p->lane_of[b], p->stream);
+    p->cur_step[b] = current_step;
+  }
+  hipStreamSynchronize(p->stream);
+  hipFree(dh);
+  hipFree(ds);
+  if (e != hipSuccess) return fail(p, TVBK_EHIP, std::string("scatter: ") + hipGetErrorString(e));
+  return TVBK_OK;
+}
+
+int tvbk_export_state(tvbk_plan* p, int32_t inst, void* history, int32_t elem_bytes, double* state,
+                      int64_t* current_step) {
+  if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
+  if (inst < 0 || inst >= p->d.n_inst) return fail(p, TVBK_EINVAL, "instance out of range");
+  if (elem_bytes != 4 && elem_bytes != 8) return fail(p, TVBK_EINVAL, "elem_bytes must be 4 or 8");
+  const TileDesc& td = p->tiles[p->tile_of[inst]];
+  const ConnDesc& cd = p->conns[td.conn];
+  const int H = cd.horizon, N = cd.n_node, nch = p->nch, nsv = p->mi.nsv;
+  const size_t nh = (size_t)H * nch * N;
+  void *dh = nullptr, *ds = nullptr;
+  HIPCHK(p, hipMalloc(&dh, nh * p->esize));
+  HIPCHK(p, hipMalloc(&ds, (size_t)nsv * N * p->esize));
+  hipError_t e = launch_scatter(p->d.precision, true, (char*)p->d_ring.p + td.ring_off * p->esize,
+                                (unsigned char*)p->d_img.p + td.img_off + p->lds_state, dh, ds, H, N, nch, nsv, p->T,
+                                p->lane_of[inst], p->stream);
+  std::vector<unsigned char> hb(nh * p->esize), sb((size_t)nsv * N * p->esize);
+  if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+  if (e == hipSuccess) e = hipMemcpy(hb.data(), dh, hb.size(), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(sb.data(), ds, sb.size(), hipMemcpyDeviceToHost);
+  hipFree(dh);
+  hipFree(ds);
+  if (e != hipSuccess) return fail(p, TVBK_EHIP, std::string("export: ") + hipGetErrorString(e));
+  if (history) {
+    std::vector<unsigned char> ho(nh * elem_bytes);
+    for (size_t k = 0; k < nh; ++k) {
+      const double v = p->esize == 8 ? reinterpret_cast<double*>(hb.data())[k] : reinterpret_cast<float*>(hb.data())[k];
+      if (elem_bytes == 8) reinterpret_cast<double*>(ho.data())[k] = v;
+      else reinterpret_cast<float*>(ho.data())[k] = (float)v;
+    }
+    HIPCHK(p, hipMemcpy(history, ho.data(), ho.size(), hipMemcpyDefault));
+  }
+  if (state) {
+    std::vector<double> so((size_t)nsv * N);
+    for (size_t k = 0; k < so.size(); ++k)
+      so[k] = p->esize == 8 ? reinterpret_cast<double*>(sb.data())[k] : reinterpret_cast<float*>(sb.data())[k];
+    HIPCHK(p, hipMemcpy(state, so.data(), so.size() * 8, hipMemcpyDefault));
+  }
+  if (current_step) *current_step = p->cur_step[inst];
+  return TVBK_OK;
+}
+
+int tvbk_init_uniform(tvbk_plan* p, uint64_t seed, const double* lo, const double* hi) {
+  if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
+  if (!lo || !hi) return fail(p, TVBK_EINVAL, "null bounds");
+  InitArgs a{};
+  a.tiles = (const TileDesc*)p->d_tiles.p;
+  a.conns = (const ConnDesc*)p->d_conns.p;
+  a.ring = p->d_ring.p;
+  a.img = (unsigned char*)p->d_img.p;
+  a.params = p->d_params.p;
+  a.streams = (const uint64_t*)p->d_streams.p;
+  a.prow = p->prow * (p->spatial ? p->conns[0].n_node : 1);
+  a.T = p->T;
+  a.lds_state = p->lds_state;
+  a.key0 = (uint32_t)seed;
+  a.key1 = (uint32_t)(seed >> 32);
+  for (int v = 0; v < p->mi.nsv; ++v) { a.lo[v] = lo[v]; a.hi[v] = hi[v]; }
+  HIPCHK(p, hipMemcpyAsync(p->d_tiles.p, p->tiles.data(), p->tiles.size() * sizeof(TileDesc), hipMemcpyHostToDevice,
+                           p->stream));
+  HIPCHK(p, launch_init(p->d.precision, p->d.model, p->d.cfun, a, (int)p->tiles.size(), p->stream));
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  std::fill(p->cur_step.begin(), p->cur_step.end(), 0);
+  return TVBK_OK;
+}
+
+int tvbk_set_noise(tvbk_plan* p, const double* z, int64_t n_steps) {
+  if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
+  if (p->d.noise != TVBK_NOISE_INJECTED) return fail(p, TVBK_ESTATE, "plan noise source is not INJECTED");
+  if (!z || n_steps <= 0) return fail(p, TVBK_EINVAL, "empty noise");
+  hipPointerAttribute_t attr;
+  bool on_device = false;
+  if (hipPointerGetAttributes(&attr, z) == hipSuccess) on_device = attr.type == hipMemoryTypeDevice;
+  (void)hipGetLastError();
+  const size_t bytes = (size_t)n_steps * p->mi.nsv * p->max_n * p->d.n_inst * 8;
+  if (on_device) {
+    p->z_ptr = z;
+  } else {
+    if (p->d_z.bytes < bytes) {
+      dev_free(&p->d_z);
+      int rc = dev_alloc(p, &p->d_z, bytes);
+      if (rc) return rc;
+    }
+    HIPCHK(p, hipMemcpyAsync(p->d_z.p, z, bytes, hipMemcpyHostToDevice, p->stream));
+    p->z_ptr = (const double*)p->d_z.p;
+  }
+  p->z_steps = n_steps;
+  return TVBK_OK;
+}
+
+int tvbk_set_steps(tvbk_plan* p, const int64_t* remaining) {
+  if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
+  for (int b = 0; b < p->d.n_inst; ++b) p->remaining[b] = remaining ? remaining[b] : -1;
+  return TVBK_OK;
+}
+
+int tvbk_run_steps(tvbk_plan* p, int64_t n) {
+  if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
+  if (n < 0) return fail(p, TVBK_EINVAL, "n_steps must be >= 0");
+  if (n > p->d.max_steps) return fail(p, TVBK_EINVAL, "n_steps exceeds the plan's max_steps");
+  const tvbk_plan_desc& d = p->d;
+  const int B = d.n_inst, T = p->T;
+  if (p->stochastic && d.noise == TVBK_NOISE_INJECTED && (p->z_ptr == nullptr || p->z_steps < n))
+    return fail(p, TVBK_ESTATE, "injected noise for this run not set (tvbk_set_noise)");
+  // the tile table is rewritten below: the previous run must have consumed it
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  // per-tile steps and monitor layout
+  int64_t out_off[TVBK_MAX_MON] = {0};
+  for (size_t k = 0; k < p->tiles.size(); ++k) {
+    TileDesc& td = p->tiles[k];
+    const int b0 = td.inst0;
+    const int64_t cs = p->cur_step[b0];
+    int64_t steps = n;
+    for (int l = 0; l < td.n_inst; ++l) {
+      const int b = b0 + l;
+      if (p->cur_step[b] != cs) return fail(p, TVBK_ESTATE, "instances of one tile must share current_step");
+      if (p->remaining[b] >= 0) steps = std::min(steps, p->remaining[b]);
+    }
+    td.step0 = cs + 1;
+    td.n_steps = steps;
+    const ConnDesc& cd = p->conns[td.conn];
+    for (int m = 0; m < d.n_mon; ++m) {
+      int64_t cnt, first;
+      emissions(td.step0, steps, d.mon[m].istep, &cnt, &first);
+      const int nv = d.mon[m].n_voi;
+      if (T > 1) {
+        td.out_off[m] = (int64_t)b0;
+        td.out_si[m] = B;
+        td.out_sv[m] = (int64_t)cd.n_node * B;
+        td.out_sj[m] = (int64_t)nv * cd.n_node * B;
+      } else {
+        td.out_off[m] = out_off[m];
+        td.out_si[m] = 1;
+        td.out_sv[m] = cd.n_node;
+        td.out_sj[m] = (int64_t)nv * cd.n_node;
+        out_off[m] += cnt * nv * cd.n_node;
+      }
+      for (int l = 0; l < td.n_inst; ++l) {
+        p->mon_count[m][b0 + l] = cnt;
+        p->mon_first[m][b0 + l] = first;
+        p->mon_offset[m][b0 + l] = (T > 1) ? 0 : td.out_off[m];
+      }
+    }
+    // injected noise [k][sv][node][inst] over max_n nodes
+    td.z_si = B;
+    td.z_ssv = (int64_t)p->max_n * B;
+    td.z_sk = (int64_t)p->mi.nsv * p->max_n * B;
+    td.z_off = b0;
+  }
+  for (int m = 0; m < d.n_mon; ++m) {
+    const int es = d.mon[m].kind == TVBK_MON_BOLD ? 8 : p->esize;
+    int64_t cnt = 0;
+    if (T > 1) cnt = p->mon_count[m][0] * d.mon[m].n_voi * (int64_t)p->conns[0].n_node * B;
+    else cnt = out_off[m];
+    p->mon_bytes[m] = cnt * es;
+  }
+  HIPCHK(p, hipMemcpyAsync(p->d_tiles.p, p->tiles.data(), p->tiles.size() * sizeof(TileDesc), hipMemcpyHostToDevice,
+                           p->stream));
+  HIPCHK(p, hipMemsetAsync(p->d_flags.p, 0, (size_t)B * 4, p->stream));
+  StepArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.tiles = (const TileDesc*)p->d_tiles.p;
+  a.conns = (const ConnDesc*)p->d_conns.p;
+  a.edges = (const uint32_t*)p->d_edges.p;
+  a.weights = p->d_w.p;
+  a.row_ptr = (const int32_t*)p->d_rp.p;
+  a.ring = p->d_ring.p;
+  a.img = (unsigned char*)p->d_img.p;
+  a.bacc = (double*)p->d_bacc.p;
+  for (int m = 0; m < d.n_mon; ++m) a.out[m] = p->d_out[m].p;
+  a.z = p->z_ptr;
+  a.params = p->d_params.p;
+  a.flags = (int32_t*)p->d_flags.p;
+  a.streams = (const uint64_t*)p->d_streams.p;
+  a.prow = p->prow;
+  a.spatial = p->spatial;
+  a.T = T;
+  a.n_mon = d.n_mon;
+  a.scheme = d.scheme;
+  a.noise = d.noise;
+  a.csr_in_lds = p->csr_in_lds ? 1 : 0;
+  int zero_delay = 0;
+  for (const ConnDesc& cd : p->conns) zero_delay |= (cd.d_min == 0 && cd.nnz > 0);
+  a.zero_delay = zero_delay;
+  a.key0 = (uint32_t)d.seed;
+  a.key1 = (uint32_t)(d.seed >> 32);
+  a.dt = d.dt;
+  a.sqrt_dt = std::sqrt(d.dt);
+  a.lds_bold = p->lds_bold;
+  a.lds_state = p->lds_state;
+  a.lds_c = p->lds_c;
+  a.lds_edge = p->lds_edge;
+  a.lds_w = p->lds_w;
+  a.lds_rp = p->lds_rp;
+  a.img_bytes = p->img_bytes;
+  for (int o = 0; o < d.n_obs; ++o) {
+    int single = -1, nz = 0;
+    for (int v = 0; v < p->mi.nsv; ++v) {
+      const double c = d.obs_coef[o * p->mi.nsv + v];
+      a.obs_coef[o][v] = c;
+      if (c != 0.0) { ++nz; if (c == 1.0) single = v; }
+    }
+    a.obs_sv[o] = (nz == 1 && single >= 0) ? single : -1;
+  }
+  for (int m = 0; m < d.n_mon; ++m) {
+    const tvbk_monitor& mo = d.mon[m];
+    MonArgs& ma = a.mon[m];
+    ma.kind = mo.kind;
+    ma.istep = mo.istep;
+    ma.nvoi = mo.n_voi;
+    for (int v = 0; v < mo.n_voi; ++v) ma.voi[v] = mo.voi[v];
+    ma.ii = mo.kind == TVBK_MON_BOLD ? mo.interim_istep : 1;
+    ma.S = mo.n_taps;
+    ma.Q = p->mon_Q[m] > 0 ? p->mon_Q[m] : 1;
+    ma.sub1 = mo.subtract_one;
+    ma.k1v0 = mo.k1_v0;
+    ma.taps = (const double*)p->d_taps[m].p;
+    ma.lds_off = p->mon_lds_off[m];
+  }
+  HIPCHK(p, hipEventRecord(p->ev0, p->stream));
+  HIPCHK(p, launch_step(d.precision, d.model, d.cfun, a, (int)p->tiles.size(), p->nthreads, (size_t)p->lds_total,
+                        p->stream));
+  HIPCHK(p, hipEventRecord(p->ev1, p->stream));
+  p->have_timing = true;
+  for (size_t k = 0; k < p->tiles.size(); ++k) {
+    const TileDesc& td = p->tiles[k];
+    for (int l = 0; l < td.n_inst; ++l) {
+      const int b = td.inst0 + l;
+      p->cur_step[b] += td.n_steps;
+      if (p->remaining[b] >= 0) p->remaining[b] -= td.n_steps;
+    }
+  }
+  p->last_n = n;
+  if (d.noise == TVBK_NOISE_INJECTED) p->z_steps = 0;
+  return TVBK_OK;
+}
+
+int tvbk_sync(tvbk_plan* p) {
+  if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  return TVBK_OK;
+}
+
+int tvbk_last_run_ms(tvbk_plan* p, double* ms) {
+  if (!p || !ms) return fail(p, TVBK_EINVAL, "null argument");
+  *ms = 0;
+  if (!p->have_timing) return TVBK_OK;
+  HIPCHK(p, hipEventSynchronize(p->ev1));
+  float f = 0;
+  HIPCHK(p, hipEventElapsedTime(&f, p->ev0, p->ev1));
+  *ms = f;
+  return TVBK_OK;
+}
+
+int tvbk_monitor_layout(tvbk_plan* p, int32_t mon, int32_t* layout, int64_t* total_bytes, int64_t* offset,
+                        int64_t* count, int64_t* first_step) {
+  if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
+  if (mon < 0 || mon >= p->d.n_mon) return fail(p, TVBK_EINVAL, "monitor out of range");
+  if (layout) *layout = p->T > 1 ? 0 : 1;
+  if (total_bytes) *total_bytes = p->mon_bytes[mon];
+  for (int b = 0; b < p->d.n_inst; ++b) {
+    if (offset) offset[b] = p->mon_offset[mon][b];
+    if (count) count[b] = p->mon_count[mon][b];
+    if (first_step) first_step[b] = p->mon_first[mon][b];
+  }
+  return TVBK_OK;
+}
+
+int tvbk_read_monitor(tvbk_plan* p, int32_t mon, void* out, int64_t cap_bytes) {
+  if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
+  if (mon < 0 || mon >= p->d.n_mon) return fail(p, TVBK_EINVAL, "monitor out of range");
+  if (cap_bytes < p->mon_bytes[mon]) return fail(p, TVBK_EINVAL, "output buffer too small");
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  if (p->mon_bytes[mon]) HIPCHK(p, hipMemcpy(out, p->d_out[mon].p, p->mon_bytes[mon], hipMemcpyDefault));
+  return TVBK_OK;
+}
+
+int tvbk_monitor_device_ptr(tvbk_plan* p, int32_t mon, void** ptr, int64_t* bytes) {
+  if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
+  if (mon < 0 || mon >= p->d.n_mon) return fail(p, TVBK_EINVAL, "monitor out of range");
+  if (ptr) *ptr = p->d_out[mon].p;
+  if (bytes) *bytes = p->mon_bytes[mon];
+  return TVBK_OK;
+}
+
+int tvbk_read_flags(tvbk_plan* p, int32_t* flags) {
+  if (!p || !flags) return fail(p, TVBK_EINVAL, "null argument");
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  HIPCHK(p, hipMemcpy(flags, p->d_flags.p, (size_t)p->d.n_inst * 4, hipMemcpyDefault));
+  return TVBK_OK;
+}
+
+int tvbk_debug_slots(tvbk_plan* p, int32_t conn, int64_t step, int32_t* slots) {
+  if (!p || !slots) return fail(p, TVBK_EINVAL, "null argument");
+  if (conn < 0 || conn >= (int)p->conns.size()) return fail(p, TVBK_EINVAL, "connectome out of range");
+  if (step < 1) return fail(p, TVBK_EINVAL, "step must be >= 1");
+  const ConnDesc& cd = p->conns[conn];
+  void* dout = nullptr;
+  HIPCHK(p, hipMalloc(&dout, (size_t)std::max(cd.nnz, 1) * 4));
+  hipError_t e = launch_debug_slots((const uint32_t*)p->d_edges.p + cd.e_off, cd.nnz, cd.horizon, step,
+                                    (int32_t*)dout, p->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+  if (e == hipSuccess && cd.nnz) e = hipMemcpy(slots, dout, (size_t)cd.nnz * 4, hipMemcpyDefault);
+  hipFree(dout);
+  if (e != hipSuccess) return fail(p, TVBK_EHIP, std::string("debug slots: ") + hipGetErrorString(e));
+  return TVBK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,265 @@
+// Device-side building blocks of the step kernels: neural-mass model dfuns,
+// integrator schemes, the history-channel (pre-synaptic) transform, the
+// numpy-order row reduction and the Philox4x32-10 noise source.
+//
+// Arithmetic follows the reference's operation order term by term (the host
+// derives the parameter combinations the reference evaluates first, e.g.
+// d*tau or 1/tau, in float64 exactly as numpy does).  The library is built
+// with -ffp-contract=off so no multiply-add is fused where the reference
+// rounds twice.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <float.h>
+
+namespace tvbk {
+
+enum { CF_LINEAR = 0, CF_SIGMOID_JR = 1, CF_IDENTITY = 2 };
+enum { SCH_EULER_DET = 0, SCH_EULER_STO = 1, SCH_HEUN_DET = 2, SCH_HEUN_STO = 3, SCH_IDENTITY = 4 };
+enum { NOISE_NONE = 0, NOISE_INJECTED = 1, NOISE_PHILOX = 2 };
+enum { MON_RAW = 0, MON_TAVG = 1, MON_BOLD = 2 };
+
+// ---------------------------------------------------------------------------
+// Models.  NDP = number of derived parameters the host prepares per instance.
+// ---------------------------------------------------------------------------
+
+// Generic2dOscillator, _numba_dfun_g2d (models/oscillator.py:385-395).
+// derived: d*tau, alpha, f, e, g, gamma*I, gamma, a, b, c, d, beta, tau
+struct G2D {
+  static constexpr int NSV = 2, NCV = 1, NDP = 13;
+  static constexpr int cvar(int k) { return 0; }
+  template <class R>
+  __device__ static inline void dfun(const R* x, const R* c, const R* p, R* dx) {
+    const R V = x[0], W = x[1];
+    const R V2 = V * V;
+    dx[0] = p[0] * (((((p[1] * W - p[2] * V2 * V) + p[3] * V2) + p[4] * V) + p[5]) + p[6] * c[0]);
+    dx[1] = (p[10] * (((p[7] + p[8] * V) + p[9] * V2) - p[11] * W)) / p[12];
+  }
+  template <class R> __device__ static inline void bound(R* x) {}
+};
+
+// MontbrioPazoRoxin.dfun (models/infinite_theta.py:171-207), bound r >= 0
+// (infinite_theta.py:129-134, applied by integrators.py:181-219).
+// derived: 1/tau, Delta/(pi*tau), pi^2*tau^2, eta, J*tau, I, cr, cv
+struct MPR {
+  static constexpr int NSV = 2, NCV = 2, NDP = 8;
+  static constexpr int cvar(int k) { return k; }
+  template <class R>
+  __device__ static inline void dfun(const R* x, const R* c, const R* p, R* dx) {
+    const R r = x[0], V = x[1];
+    dx[0] = p[0] * (p[1] + (R(2) * V) * r);
+    dx[1] = p[0] * ((((((V * V - p[2] * (r * r)) + p[3]) + p[4] * r) + p[5]) + p[6] * c[0]) + p[7] * c[1]);
+  }
+  template <class R> __device__ static inline void bound(R* x) {
+    if (x[0] < R(0)) x[0] = R(0);
+    if (x[0] > R(DBL_MAX)) x[0] = R(DBL_MAX);
+  }
+};
+
+// JansenRit, _numba_dfun_jr (models/jansen_rit.py:276-289).  Coupling enters
+// through row 0 only (jansen_rit.py:268, :288).
+// derived: 2*nu_max, r, v0, a_1*J, a_3*J, A*a, 2*a, a^2, mu, a_2*J, B*b,
+//          a_4*J, 2*b, b^2
+struct JR {
+  static constexpr int NSV = 6, NCV = 2, NDP = 14;
+  static constexpr int cvar(int k) { return 1 + k; }
+  template <class R>
+  __device__ static inline void dfun(const R* y, const R* c, const R* p, R* dx) {
+    const R one = R(1);
+    const R s12 = p[0] / (one + exp(p[1] * (p[2] - (y[1] - y[2]))));
+    const R s01 = p[0] / (one + exp(p[1] * (p[2] - p[3] * y[0])));
+    const R s03 = p[0] / (one + exp(p[1] * (p[2] - p[4] * y[0])));
+    dx[0] = y[3];
+    dx[1] = y[4];
+    dx[2] = y[5];
+    dx[3] = (p[5] * s12 - p[6] * y[3]) - p[7] * y[0];
+    dx[4] = (p[5] * ((p[8] + p[9] * s01) + c[0]) - p[6] * y[4]) - p[7] * y[1];
+    dx[5] = (p[10] * (p[11] * s03) - p[12] * y[5]) - p[13] * y[2];
+  }
+  template <class R> __device__ static inline void bound(R* x) {}
+};
+
+// Sum: dx = x + c, the integer test model of history_test.py:55-65.
+struct SUM {
+  static constexpr int NSV = 1, NCV = 1, NDP = 0;
+  static constexpr int cvar(int k) { return 0; }
+  template <class R>
+  __device__ static inline void dfun(const R* x, const R* c, const R* p, R* dx) { dx[0] = x[0] + c[0]; }
+  template <class R> __device__ static inline void bound(R* x) {}
+};
+
+// ---------------------------------------------------------------------------
+// Coupling.  History channels hold what the coupling sums: the cvars for
+// Linear/IdCoupling, and the pre-synaptic sigmoid of (y1 - y2) for
+// SigmoidalJansenRit, whose pre depends on the source node only
+// (coupling.py:370-373), so it is applied once when the state is written.
+// NCH = history channels, NCT = coupling terms handed to the dfun.
+// derived coupling params: Linear a, b; SigmoidalJR cmin, cmax-cmin, r,
+// midpoint, a.
+// ---------------------------------------------------------------------------
+template <class M, int CF> struct Cfun;
+template <class M> struct Cfun<M, CF_LINEAR> {
+  static constexpr int NCH = M::NCV, NCT = M::NCV, NCP = 2;
+  template <class R> __device__ static inline R channel(const R* x, int ch, const R* cp) { return x[M::cvar(ch)]; }
+  template <class R> __device__ static inline R post(R gx, const R* cp) { return cp[0] * gx + cp[1]; }
+};
+template <class M> struct Cfun<M, CF_IDENTITY> {
+  static constexpr int NCH = M::NCV, NCT = M::NCV, NCP = 0;
+  template <class R> __device__ static inline R channel(const R* x, int ch, const R* cp) { return x[M::cvar(ch)]; }
+  template <class R> __device__ static inline R post(R gx, const R* cp) { return gx; }
+};
+template <class M> struct Cfun<M, CF_SIGMOID_JR> {
+  static constexpr int NCH = 1, NCT = 1, NCP = 5;
+  // SigmoidalJansenRit.pre (coupling.py:370-373) on x_j[:,0] - x_j[:,1]
+  template <class R> __device__ static inline R channel(const R* x, int ch, const R* cp) {
+    return cp[0] + cp[1] / (R(1) + exp(cp[2] * (cp[3] - (x[M::cvar(0)] - x[M::cvar(1)]))));
+  }
+  template <class R> __device__ static inline R post(R gx, const R* cp) { return cp[4] * gx; }
+};
+
+// ---------------------------------------------------------------------------
+// numpy add.reduceat order for one segment: x[0] + pairwise_sum(x[1:]),
+// pairwise_sum as in numpy's loops: n < 8 sequential from 0, n <= 128 eight
+// accumulators combined ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)) then the tail,
+// larger n split at a multiple of 8 near n/2.  Handles n <= 1025.
+// ---------------------------------------------------------------------------
+template <class R, class F>
+__device__ inline R pw_leaf(int s, int n, F& f) {
+  if (n < 8) {
+    R res = R(0);
+    for (int i = 0; i < n; ++i) res += f(s + i);
+    return res;
+  }
+  R r0 = f(s), r1 = f(s + 1), r2 = f(s + 2), r3 = f(s + 3);
+  R r4 = f(s + 4), r5 = f(s + 5), r6 = f(s + 6), r7 = f(s + 7);
+  const int nfull = n - (n & 7);
+  int i = 8;
+  for (; i < nfull; i += 8) {
+    r0 += f(s + i);     r1 += f(s + i + 1);
+    r2 += f(s + i + 2); r3 += f(s + i + 3);
+    r4 += f(s + i + 4); r5 += f(s + i + 5);
+    r6 += f(s + i + 6); r7 += f(s + i + 7);
+  }
+  R res = ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7));
+  for (; i < n; ++i) res += f(s + i);
+  return res;
+}
+
+template <int DEPTH, class R, class F>
+__device__ inline R pw_sum(int s, int n, F& f) {
+  if constexpr (DEPTH == 0) {
+    return pw_leaf<R>(s, n, f);
+  } else {
+    if (n <= 128) return pw_leaf<R>(s, n, f);
+    int n2 = n / 2;
+    n2 -= n2 & 7;
+    R a = pw_sum<DEPTH - 1, R>(s, n2, f);
+    R b = pw_sum<DEPTH - 1, R>(s + n2, n - n2, f);
+    return a + b;
+  }
+}
+
+template <class R, class F>
+__device__ inline R reduceat_row(int n, F& f) {
+  if (n <= 0) return R(0);
+  R first = f(0);
+  if (n == 1) return first;
+  return first + pw_sum<3, R>(1, n - 1, f);
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11) and Box-Muller normals.
+// ---------------------------------------------------------------------------
+struct U4 { uint32_t x, y, z, w; };
+
+__device__ inline U4 philox10(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+    c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+  }
+  return c;
+}
+
+__device__ inline void box_muller(uint32_t a, uint32_t b, double* z0, double* z1) {
+  const double u1 = ((double)a + 1.0) * 2.3283064365386963e-10;   // 2^-32
+  const double u2 = (double)b * 2.3283064365386963e-10;
+  const double rad = sqrt(-2.0 * log(u1));
+  double s, c;
+  sincos(2.0 * 3.141592653589793 * u2, &s, &c);
+  *z0 = rad * c;
+  *z1 = rad * s;
+}
+
+__device__ inline void box_muller(uint32_t a, uint32_t b, float* z0, float* z1) {
+  const float u1 = (float)((a >> 8) + 1u) * 5.9604644775390625e-8f;  // 2^-24
+  const float u2 = (float)(b >> 8) * 5.9604644775390625e-8f;
+  const float rad = sqrtf(-2.0f * logf(u1));
+  float s, c;
+  sincosf(6.2831853071795865f * u2, &s, &c);
+  *z0 = rad * c;
+  *z1 = rad * s;
+}
+
+// Standard normals for state variables 0..nsv-1 of (node, stream, step).
+template <class R>
+__device__ inline void philox_normals(int nsv, uint32_t node, uint64_t stream, int64_t step,
+                                      uint32_t k0, uint32_t k1, R* z) {
+  for (int call = 0; call * 4 < nsv; ++call) {
+    U4 c{node | ((uint32_t)call << 20), (uint32_t)stream, (uint32_t)(uint64_t)step,
+         (uint32_t)((uint64_t)step >> 32)};
+    U4 w = philox10(c, k0, k1);
+    R a, b, cc, d;
+    box_muller(w.x, w.y, &a, &b);
+    box_muller(w.z, w.w, &cc, &d);
+    const int o = call * 4;
+    if (o + 0 < nsv) z[o + 0] = a;
+    if (o + 1 < nsv) z[o + 1] = b;
+    if (o + 2 < nsv) z[o + 2] = cc;
+    if (o + 3 < nsv) z[o + 3] = d;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Integrator schemes (integrators.py:286-426, :496-508).  xi[sv] is the noise
+// increment sqrt(dt)*z*gfun of this step (zero for deterministic schemes).
+// ---------------------------------------------------------------------------
+template <class R, class M>
+__device__ inline void scheme_step(int sch, R* x, const R* c, const R* p, R dt, const R* xi) {
+  constexpr int S = M::NSV;
+  R m[S], y[S], m2[S];
+  M::dfun(x, c, p, m);
+  switch (sch) {
+    case SCH_HEUN_DET:
+    case SCH_HEUN_STO: {
+      const bool sto = (sch == SCH_HEUN_STO);
+#pragma unroll
+      for (int k = 0; k < S; ++k) y[k] = sto ? (x[k] + dt * m[k]) + xi[k] : x[k] + dt * m[k];
+      M::template bound<R>(y);
+      M::dfun(y, c, p, m2);
+#pragma unroll
+      for (int k = 0; k < S; ++k) {
+        const R dX = ((m[k] + m2[k]) * dt) * R(0.5);
+        x[k] = sto ? (x[k] + dX) + xi[k] : x[k] + dX;
+      }
+      break;
+    }
+    case SCH_EULER_DET:
+#pragma unroll
+      for (int k = 0; k < S; ++k) x[k] = x[k] + dt * m[k];
+      break;
+    case SCH_EULER_STO:
+#pragma unroll
+      for (int k = 0; k < S; ++k) x[k] = (x[k] + m[k] * dt) + xi[k];
+      break;
+    default:  // identity
+#pragma unroll
+      for (int k = 0; k < S; ++k) x[k] = m[k];
+      break;
+  }
+  M::template bound<R>(x);
+}
+
+}  // namespace tvbk

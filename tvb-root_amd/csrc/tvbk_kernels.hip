@@ -1,0 +1,434 @@
+// Persistent fused step kernel for TVB's simulator loop on gfx950.
+//
+// One workgroup owns a "tile": T simulation instances that share one
+// connectome (T = 16/32/64 for parameter sweeps over a shared connectome;
+// T = 1 for instances with their own connectome, one per workgroup).  The
+// workgroup advances its tile through all steps of a call without any
+// inter-workgroup synchronisation: instances are independent.
+//
+// Per step s (Simulator.__call__, simulator.py:356-363, with the coupling of
+// step s evaluated from the history as it stood after step s-1):
+//   Phase A  coupling c(s): every (target node, history channel) pair is
+//            reduced by a group of T lanes (one lane per instance) over the
+//            node's CSR row in numpy add.reduceat order, gathering
+//            history[(s-1-d) % H][source][channel][lane] -- T consecutive
+//            values per edge, so each gather is one coalesced segment.
+//   barrier
+//   Phase B  integrator step of every (node, instance) with c(s), write of
+//            history slot s % H, on-device monitors.
+// With every delay >= 1, phase A of step s+1 only reads slots written before
+// the barrier of step s, so one barrier per step suffices; coupling terms are
+// double-buffered by step parity.  With zero delays a second barrier orders
+// the history write before the next gather.
+//
+// Layouts (elements of the plan precision R unless stated):
+//   history  ring[tile][slot][node][channel][T]
+//   LDS      bold interim (f64) | state [node][svar][T] | tavg sum, comp
+//            [node][voi][T] | c[2][node][term][T] | CSR (packed col|delay,
+//            weight, row_ptr) when it fits
+//   the persistent part of LDS (bold interim, state, tavg) is copied from and
+//   back to a per-tile image in HBM around each call.
+#include "tvbk_kernels.hpp"
+#include "tvbk_device.hpp"
+
+namespace tvbk {
+
+template <class R>
+__device__ inline void copy_bytes(unsigned char* dst, const unsigned char* src, int64_t n) {
+  // 16-byte granularity copy; all images are padded to 16 bytes.
+  const int64_t n16 = n >> 4;
+  uint4* d = reinterpret_cast<uint4*>(dst);
+  const uint4* s = reinterpret_cast<const uint4*>(src);
+  for (int64_t k = threadIdx.x; k < n16; k += blockDim.x) d[k] = s[k];
+}
+
+template <class R, class M, int CF>
+__global__ void __launch_bounds__(1024) k_step(StepArgs a) {
+  using C = Cfun<M, CF>;
+  constexpr int NSV = M::NSV, NCH = C::NCH, NCT = C::NCT, NDP = M::NDP, NCP = C::NCP;
+  extern __shared__ __align__(16) unsigned char smem[];
+
+  const TileDesc td = a.tiles[blockIdx.x];
+  if (td.n_steps <= 0) return;
+  const ConnDesc cd = a.conns[td.conn];
+  const int T = a.T;
+  const int N = cd.n_node, H = cd.horizon;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6, nwave = nthr >> 6;
+
+  // ---- LDS carve ---------------------------------------------------------
+  unsigned char* img = smem;                              // persistent image
+  double* s_bi = reinterpret_cast<double*>(smem + a.lds_bold);
+  R* s_x = reinterpret_cast<R*>(smem + a.lds_state);
+  R* s_c = reinterpret_cast<R*>(smem + a.lds_c);
+  const uint32_t* g_edge = a.edges + cd.e_off;
+  const R* g_w = reinterpret_cast<const R*>(a.weights) + cd.e_off;
+  const int32_t* g_rp = a.row_ptr + cd.rp_off;
+  const uint32_t* e_edge = g_edge;
+  const R* e_w = g_w;
+  const int32_t* e_rp = g_rp;
+  const int nnz = cd.nnz;
+  if (a.csr_in_lds) {
+    uint32_t* l_edge = reinterpret_cast<uint32_t*>(smem + a.lds_edge);
+    R* l_w = reinterpret_cast<R*>(smem + a.lds_w);
+    int32_t* l_rp = reinterpret_cast<int32_t*>(smem + a.lds_rp);
+    for (int k = tid; k < nnz; k += nthr) { l_edge[k] = g_edge[k]; l_w[k] = g_w[k]; }
+    for (int k = tid; k <= N; k += nthr) l_rp[k] = g_rp[k];
+    e_edge = l_edge; e_w = l_w; e_rp = l_rp;
+  }
+  copy_bytes<R>(img, a.img + td.img_off, a.img_bytes);
+  __syncthreads();
+
+  // ---- per-lane instance parameters (phase B lane and phase A lane agree:
+  //      blockDim and 64 are multiples of T) -------------------------------
+  const int t = tid % T;
+  const bool live = t < td.n_inst;
+  const int inst = td.inst0 + (live ? t : 0);
+  const int64_t spatial_row = (int64_t)N * a.prow;   // per-node params: one prow row per node
+  const R* prow = reinterpret_cast<const R*>(a.params) + (int64_t)inst * (a.spatial ? spatial_row : a.prow);
+  R cp[NCP > 0 ? NCP : 1];
+#pragma unroll
+  for (int k = 0; k < NCP; ++k) cp[k] = prow[NDP + k];
+  R gf[NSV];
+#pragma unroll
+  for (int k = 0; k < NSV; ++k) gf[k] = (a.noise != NOISE_NONE) ? prow[NDP + NCP + k] : R(0);
+  R pm[NDP > 0 ? NDP : 1];
+#pragma unroll
+  for (int k = 0; k < NDP; ++k) pm[k] = prow[k];
+  const R dt = (R)a.dt;
+  const uint64_t stream = a.streams ? a.streams[inst] : (uint64_t)inst;
+
+  R* ring = reinterpret_cast<R*>(a.ring) + td.ring_off;
+  const int rstride = NCH * T;   // elements per (slot, node)
+
+  // ---- step counters (wave-uniform) --------------------------------------
+  int64_t s = td.step0;
+  int smod = (int)((s - 1) % H);          // (s-1) % H ; s >= 1
+  int mcnt[TVBK_KMAX_MON], icnt[TVBK_KMAX_MON];
+  int64_t msample[TVBK_KMAX_MON], mpush[TVBK_KMAX_MON];
+#pragma unroll
+  for (int m = 0; m < TVBK_KMAX_MON; ++m) {
+    if (m < a.n_mon) {
+      mcnt[m] = (int)(s % a.mon[m].istep);
+      icnt[m] = a.mon[m].kind == MON_BOLD ? (int)(s % a.mon[m].ii) : 0;
+      msample[m] = 0;
+      mpush[m] = a.mon[m].kind == MON_BOLD ? s / a.mon[m].ii : 0;
+    }
+  }
+  int bad = 0;
+  const int P = N * NCH;
+  const int NG = 64 / T;        // lane groups per wave
+  const int g = lane / T;
+
+  for (int64_t k = 0; k < td.n_steps; ++k, ++s) {
+    const int buf = (int)(k & 1);
+    R* cbuf = s_c + (int64_t)buf * N * NCT * T;
+    // ================= phase A: coupling of step s ========================
+    for (int q = wave * NG + g; q - g < P; q += nwave * NG) {
+      if (q < P) {
+        const int i = q / NCH, ch = q - (q / NCH) * NCH;
+        const int e0 = e_rp[i];
+        const int n = e_rp[i + 1] - e0;
+        const R* rb = ring + ch * T + t;
+        auto prod = [&](int e) -> R {
+          const uint32_t ed = e_edge[e0 + e];
+          const int j = (int)(ed & 0xffffu), d = (int)(ed >> 16);
+          int sl = smod - d;
+          sl += (sl < 0) ? H : 0;
+          return e_w[e0 + e] * rb[(sl * N + j) * rstride];
+        };
+        R acc = reduceat_row<R>(n, prod);
+        if (CF == CF_SIGMOID_JR) {
+          cbuf[(i * NCT) * T + t] = C::post(acc, cp);
+        } else {
+          cbuf[(i * NCT + ch) * T + t] = C::post(acc, cp);
+        }
+      }
+    }
+    __syncthreads();
+    // ================= phase B: integrate, write history, monitors =========
+    const int wslot = (smod + 1 == H) ? 0 : smod + 1;   // s % H
+    for (int i = tid / T; i < N; i += nthr / T) {
+      R x[NSV], c[NCT], xi[NSV];
+#pragma unroll
+      for (int v = 0; v < NSV; ++v) x[v] = s_x[(i * NSV + v) * T + t];
+#pragma unroll
+      for (int v = 0; v < NCT; ++v) c[v] = cbuf[(i * NCT + v) * T + t];
+      const R* pp = pm;
+      R pn[NDP > 0 ? NDP : 1];
+      if (a.spatial) {
+        const R* pr = reinterpret_cast<const R*>(a.params) + (int64_t)inst * spatial_row + (int64_t)i * a.prow;
+#pragma unroll
+        for (int v = 0; v < NDP; ++v) pn[v] = pr[v];
+#pragma unroll
+        for (int v = 0; v < NSV; ++v) gf[v] = (a.noise != NOISE_NONE) ? pr[NDP + NCP + v] : R(0);
+        pp = pn;
+      }
+      if (a.noise == NOISE_INJECTED) {
+        const double* z = a.z + td.z_off + k * td.z_sk + (int64_t)i * td.z_si + t;
+#pragma unroll
+        for (int v = 0; v < NSV; ++v) xi[v] = (R)((a.sqrt_dt * z[v * td.z_ssv]) * (double)gf[v]);
+      } else if (a.noise == NOISE_PHILOX) {
+        R zz[NSV];
+        philox_normals<R>(NSV, (uint32_t)i, stream, s, a.key0, a.key1, zz);
+#pragma unroll
+        for (int v = 0; v < NSV; ++v) xi[v] = ((R)a.sqrt_dt * zz[v]) * gf[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < NSV; ++v) xi[v] = R(0);
+      }
+      scheme_step<R, M>(a.scheme, x, c, pp, dt, xi);
+#pragma unroll
+      for (int v = 0; v < NSV; ++v) {
+        bad |= !isfinite(x[v]);
+        s_x[(i * NSV + v) * T + t] = x[v];
+      }
+      R* wr = ring + (wslot * N + i) * rstride + t;
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) wr[ch * T] = C::template channel<R>(x, ch, cp);
+
+      // ---------------- monitors (monitors.py:160-190, :363-391, :834-962)
+#pragma unroll
+      for (int m = 0; m < TVBK_KMAX_MON; ++m) {
+        if (m >= a.n_mon) break;
+        const MonArgs& mo = a.mon[m];
+        const int nv = mo.nvoi;
+        const bool emit = (mcnt[m] == 0);
+        if (mo.kind == MON_RAW || mo.kind == MON_TAVG) {
+          R* out = reinterpret_cast<R*>(a.out[m]) + td.out_off[m] + msample[m] * td.out_sj[m] +
+                   (int64_t)i * td.out_si[m] + t;
+          R* ts = reinterpret_cast<R*>(smem + mo.lds_off);
+          for (int v = 0; v < nv; ++v) {
+            const int o = mo.voi[v];
+            R val;
+            if (a.obs_sv[o] >= 0) {
+              val = x[a.obs_sv[o]];
+            } else {
+              val = R(0);
+#pragma unroll
+              for (int u = 0; u < NSV; ++u) val = val + (R)a.obs_coef[o][u] * x[u];
+            }
+            if (mo.kind == MON_RAW) {
+              if (live) out[v * td.out_sv[m]] = val;
+            } else {
+              // running sum over the averaging window: Kahan-compensated in
+              // float32, plain sequential (numpy's mean order) in float64
+              R* sum = ts + ((i * nv + v) * 2 + 0) * T + t;
+              R* cmp = ts + ((i * nv + v) * 2 + 1) * T + t;
+              R tv;
+              if constexpr (sizeof(R) == 8) {
+                tv = *sum + val;
+              } else {
+                const R yv = val - *cmp;
+                tv = *sum + yv;
+                *cmp = (tv - *sum) - yv;
+              }
+              *sum = tv;
+              if (emit) {
+                if (live) out[v * td.out_sv[m]] = tv / (R)mo.istep;
+                *sum = R(0);
+                *cmp = R(0);
+              }
+            }
+          }
+        } else {  // Bold: interim mean every ii steps pushed into FIR accumulators
+          double* bi = reinterpret_cast<double*>(smem + mo.lds_off);
+          const bool push = (icnt[m] == 0);
+          double* acc = a.bacc + td.bacc_off[m];
+          for (int v = 0; v < nv; ++v) {
+            const int o = mo.voi[v];
+            R val;
+            if (a.obs_sv[o] >= 0) {
+              val = x[a.obs_sv[o]];
+            } else {
+              val = R(0);
+#pragma unroll
+              for (int u = 0; u < NSV; ++u) val = val + (R)a.obs_coef[o][u] * x[u];
+            }
+            double* bsum = bi + (i * nv + v) * T + t;
+            *bsum = *bsum + (double)val;
+            if (push) {
+              const double mean = *bsum / (double)mo.ii;
+              *bsum = 0.0;
+              // sample m' = mpush contributes to emissions kk with
+              // kk*istep >= m'*ii and kk*istep < (m'+S)*ii; weight by age
+              // a = floor(kk*istep/ii) - m': a == 0 -> G[S-1], else G[a-1]
+              const int64_t mp = mpush[m];
+              const int64_t klo = (mp * mo.ii + mo.istep - 1) / mo.istep;
+              const int64_t khi = ((mp + mo.S) * mo.ii - 1) / mo.istep;
+              for (int64_t kk = klo; kk <= khi; ++kk) {
+                const int64_t age = (kk * mo.istep) / mo.ii - mp;
+                const double w = mo.taps[age == 0 ? mo.S - 1 : age - 1];
+                double* ap = acc + (((kk % mo.Q) * nv + v) * (int64_t)N + i) * T + t;
+                *ap = *ap + w * mean;
+              }
+            }
+            if (emit) {
+              const int64_t kk = s / mo.istep;
+              double* ap = acc + (((kk % mo.Q) * nv + v) * (int64_t)N + i) * T + t;
+              double* out = reinterpret_cast<double*>(a.out[m]) + td.out_off[m] + msample[m] * td.out_sj[m] +
+                            (int64_t)v * td.out_sv[m] + (int64_t)i * td.out_si[m] + t;
+              const double bold = mo.sub1 ? (*ap - 1.0) * mo.k1v0 : *ap;
+              if (live) *out = bold;
+              *ap = 0.0;
+            }
+          }
+        }
+      }
+    }
+    // ---- advance counters
+    smod = wslot;
+#pragma unroll
+    for (int m = 0; m < TVBK_KMAX_MON; ++m) {
+      if (m >= a.n_mon) break;
+      if (mcnt[m] == 0) msample[m] += 1;
+      mcnt[m] = (mcnt[m] + 1 == a.mon[m].istep) ? 0 : mcnt[m] + 1;
+      if (a.mon[m].kind == MON_BOLD) {
+        if (icnt[m] == 0) mpush[m] += 1;
+        icnt[m] = (icnt[m] + 1 == a.mon[m].ii) ? 0 : icnt[m] + 1;
+      }
+    }
+    if (a.zero_delay) __syncthreads();
+  }
+  // note: msample counts emissions of this call; the host derives the same
+  // count from the step range.
+  __syncthreads();
+  copy_bytes<R>(a.img + td.img_off, img, a.img_bytes);
+  if (bad && live) atomicOr(a.flags + inst, 1);
+}
+
+// Fills the ring and state of every tile from Philox uniforms in [lo, hi),
+// then bounds them (history.py:150-152).  The state is the value of slot
+// current_step % H = 0.
+template <class R, class M, int CF>
+__global__ void k_init_uniform(InitArgs a) {
+  using C = Cfun<M, CF>;
+  constexpr int NSV = M::NSV, NCH = C::NCH, NDP = M::NDP;
+  const TileDesc td = a.tiles[blockIdx.y];
+  const ConnDesc cd = a.conns[td.conn];
+  const int T = a.T, N = cd.n_node, H = cd.horizon;
+  const int64_t per_tile = (int64_t)H * N * T;
+  R* ring = reinterpret_cast<R*>(a.ring) + td.ring_off;
+  R* st = reinterpret_cast<R*>(a.img + td.img_off + a.lds_state);
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < per_tile; q += (int64_t)gridDim.x * blockDim.x) {
+    const int t = (int)(q % T);
+    const int i = (int)((q / T) % N);
+    const int slot = (int)(q / ((int64_t)T * N));
+    const int inst = td.inst0 + (t < td.n_inst ? t : 0);
+    const uint64_t stream = a.streams ? a.streams[inst] : (uint64_t)inst;
+    R x[NSV];
+    for (int call = 0; call * 4 < NSV; ++call) {
+      U4 c{(uint32_t)i | ((uint32_t)call << 20), (uint32_t)stream, (uint32_t)slot, 0x80000000u};
+      U4 w = philox10(c, a.key0, a.key1);
+      const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+      for (int u = 0; u < 4 && call * 4 + u < NSV; ++u) {
+        const int v = call * 4 + u;
+        const double uu = (double)ws[u] * 2.3283064365386963e-10;
+        x[v] = (R)(a.lo[v] + (a.hi[v] - a.lo[v]) * uu);
+      }
+    }
+    M::template bound<R>(x);
+    const R* prow = reinterpret_cast<const R*>(a.params) + (int64_t)inst * a.prow;
+    R cp[C::NCP > 0 ? C::NCP : 1];
+    for (int k = 0; k < C::NCP; ++k) cp[k] = prow[NDP + k];
+    for (int ch = 0; ch < NCH; ++ch) ring[((int64_t)slot * N + i) * NCH * T + ch * T + t] = C::template channel<R>(x, ch, cp);
+    if (slot == 0)
+      for (int v = 0; v < NSV; ++v) st[(i * NSV + v) * T + t] = x[v];
+  }
+}
+
+template <class R>
+__global__ void k_scatter(bool gather, R* ring, R* st, R* hist, R* state, int H, int N, int nch, int nsv, int T,
+                          int t) {
+  const int64_t nh = (int64_t)H * N * nch;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nh + (int64_t)nsv * N;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    if (q < nh) {
+      const int i = (int)(q % N), ch = (int)((q / N) % nch), slot = (int)(q / ((int64_t)N * nch));
+      R* r = ring + (((int64_t)slot * N + i) * nch + ch) * T + t;
+      if (gather) hist[q] = *r; else *r = hist[q];
+    } else {
+      const int64_t u = q - nh;
+      const int i = (int)(u % N), v = (int)(u / N);
+      R* r = st + ((int64_t)i * nsv + v) * T + t;
+      if (gather) state[u] = *r; else *r = state[u];
+    }
+  }
+}
+
+// Slot read by every edge of a connectome at `step`, with the kernels' slot
+// arithmetic.
+__global__ void k_debug_slots(const uint32_t* edges, int nnz, int H, int64_t step, int32_t* out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  const int smod = (int)((step - 1) % H);
+  const int d = (int)(edges[e] >> 16);
+  int sl = smod - d;
+  sl += (sl < 0) ? H : 0;
+  out[e] = sl;
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------
+template <class R, class M, int CF>
+static hipError_t launch_t(const StepArgs& a, int ntiles, int nthreads, size_t lds, hipStream_t st) {
+  auto kern = k_step<R, M, CF>;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(ntiles), dim3(nthreads), lds, st, a);
+  return hipGetLastError();
+}
+
+template <class R, class M, int CF>
+static hipError_t launch_init_t(const InitArgs& a, int ntiles, hipStream_t st) {
+  hipLaunchKernelGGL((k_init_uniform<R, M, CF>), dim3(256, ntiles), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+template <class R>
+static hipError_t dispatch(int model, int cf, bool init, const StepArgs* a, const InitArgs* ia, int ntiles,
+                           int nthreads, size_t lds, hipStream_t st) {
+#define TVBK_CASE(MODEL, CFV)                                                       \
+  if (init) return launch_init_t<R, MODEL, CFV>(*ia, ntiles, st);                   \
+  return launch_t<R, MODEL, CFV>(*a, ntiles, nthreads, lds, st);
+  if (model == 0 && cf == CF_LINEAR) { TVBK_CASE(G2D, CF_LINEAR) }
+  if (model == 1 && cf == CF_LINEAR) { TVBK_CASE(MPR, CF_LINEAR) }
+  if (model == 2 && cf == CF_LINEAR) { TVBK_CASE(JR, CF_LINEAR) }
+  if (model == 2 && cf == CF_SIGMOID_JR) { TVBK_CASE(JR, CF_SIGMOID_JR) }
+  if (model == 3 && cf == CF_IDENTITY) { TVBK_CASE(SUM, CF_IDENTITY) }
+  if (model == 3 && cf == CF_LINEAR) { TVBK_CASE(SUM, CF_LINEAR) }
+#undef TVBK_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_step(int precision, int model, int cf, const StepArgs& a, int ntiles, int nthreads, size_t lds,
+                       hipStream_t st) {
+  if (precision == 1) return dispatch<double>(model, cf, false, &a, nullptr, ntiles, nthreads, lds, st);
+  return dispatch<float>(model, cf, false, &a, nullptr, ntiles, nthreads, lds, st);
+}
+
+hipError_t launch_init(int precision, int model, int cf, const InitArgs& a, int ntiles, hipStream_t st) {
+  if (precision == 1) return dispatch<double>(model, cf, true, nullptr, &a, ntiles, 0, 0, st);
+  return dispatch<float>(model, cf, true, nullptr, &a, ntiles, 0, 0, st);
+}
+
+hipError_t launch_scatter(int precision, bool gather, void* ring, unsigned char* img_state, void* hist,
+                          void* state, int H, int N, int nch, int nsv, int T, int t, hipStream_t st) {
+  if (precision == 1)
+    hipLaunchKernelGGL(k_scatter<double>, dim3(64), dim3(256), 0, st, gather, (double*)ring, (double*)img_state,
+                       (double*)hist, (double*)state, H, N, nch, nsv, T, t);
+  else
+    hipLaunchKernelGGL(k_scatter<float>, dim3(64), dim3(256), 0, st, gather, (float*)ring, (float*)img_state,
+                       (float*)hist, (float*)state, H, N, nch, nsv, T, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_debug_slots(const uint32_t* edges, int nnz, int H, int64_t step, int32_t* out, hipStream_t st) {
+  if (nnz <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_debug_slots, dim3((nnz + 255) / 256), dim3(256), 0, st, edges, nnz, H, step, out);
+  return hipGetLastError();
+}
+
+}  // namespace tvbk

@@ -1,0 +1,79 @@
+// Kernel argument structures shared by the step kernels and the host API.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define TVBK_KMAX_MON 4
+#define TVBK_KMAX_OBS 8
+
+namespace tvbk {
+
+// One connectome as uploaded: CSR rows of packed (col | delay << 16) edges.
+struct ConnDesc {
+  int32_t n_node, horizon, nnz, d_min;
+  int64_t rp_off, e_off;   // element offsets into the concatenated arrays
+};
+
+// One workgroup's tile of instances and where its data lives.
+struct TileDesc {
+  int32_t conn, n_inst, inst0, pad;
+  int64_t step0, n_steps;          // first step of this call, steps to run
+  int64_t ring_off, img_off;       // ring elements, image bytes
+  int64_t bacc_off[TVBK_KMAX_MON]; // Bold accumulators (doubles)
+  int64_t out_off[TVBK_KMAX_MON], out_sj[TVBK_KMAX_MON], out_sv[TVBK_KMAX_MON], out_si[TVBK_KMAX_MON];
+  int64_t z_off, z_sk, z_ssv, z_si;  // injected noise addressing
+};
+
+struct MonArgs {
+  int32_t kind, istep, nvoi, ii, S, Q, sub1, lds_off;
+  int32_t voi[TVBK_KMAX_OBS];
+  double k1v0;
+  const double* taps;
+};
+
+struct StepArgs {
+  const TileDesc* tiles;
+  const ConnDesc* conns;
+  const uint32_t* edges;
+  const void* weights;
+  const int32_t* row_ptr;
+  void* ring;
+  unsigned char* img;
+  double* bacc;
+  void* out[TVBK_KMAX_MON];
+  const double* z;
+  const void* params;
+  int32_t* flags;
+  const uint64_t* streams;
+  int32_t prow, spatial, T, n_mon, scheme, noise, csr_in_lds, zero_delay;
+  uint32_t key0, key1;
+  double dt, sqrt_dt;
+  int32_t lds_bold, lds_state, lds_c, lds_edge, lds_w, lds_rp;
+  int64_t img_bytes;
+  int32_t obs_sv[TVBK_KMAX_OBS];
+  double obs_coef[TVBK_KMAX_OBS][8];
+  MonArgs mon[TVBK_KMAX_MON];
+};
+
+struct InitArgs {
+  const TileDesc* tiles;
+  const ConnDesc* conns;
+  void* ring;
+  unsigned char* img;
+  const void* params;
+  const uint64_t* streams;
+  int32_t prow, T, lds_state;
+  uint32_t key0, key1;
+  double lo[8], hi[8];
+};
+
+hipError_t launch_step(int precision, int model, int cf, const StepArgs& a, int ntiles, int nthreads, size_t lds,
+                       hipStream_t st);
+hipError_t launch_init(int precision, int model, int cf, const InitArgs& a, int ntiles, hipStream_t st);
+hipError_t launch_debug_slots(const uint32_t* edges, int nnz, int H, int64_t step, int32_t* out, hipStream_t st);
+// scatter a compact history [H][nch][N] / state [nsv][N] (plan precision)
+// into lane t of a tile, and the reverse gather.
+hipError_t launch_scatter(int precision, bool gather, void* ring, unsigned char* img_state, void* hist,
+                          void* state, int H, int N, int nch, int nsv, int T, int t, hipStream_t st);
+
+}  // namespace tvbk

@@ -1,0 +1,253 @@
+"""HipBackend: the TVB backend plugin that runs Simulator.__call__ on gfx950.
+
+Plugin surface as in tvb_library/tvb/simulator/backend/nb_mpr.py:79-121:
+``check_compatibility(sim)`` raises NotImplementedError for unsupported
+components, ``run_sim(sim, nstep=None, simulation_length=None, ...)`` returns
+the ``Simulator.run()`` shaped list ``[(t, x), ...]`` (one entry per monitor,
+``x.shape == (n, nvoi, N, 1)``).  ``run_batch`` runs many instances (a
+parameter sweep over one connectome, or simulators with their own
+connectomes and durations) in one launch.
+
+The configured simulator may be a ``tvb.simulator.simulator.Simulator`` or
+the mirror in :mod:`tvb_hip.sim`; components are recognised by class name.
+"""
+import numpy as np
+
+from . import _lib as L
+from .plan import CSR, MonitorSpec, Plan
+
+MODELS = {"Generic2dOscillator": L.MODEL_G2D, "MontbrioPazoRoxin": L.MODEL_MPR, "JansenRit": L.MODEL_JR}
+MODEL_PARAMS = {
+    # raw parameter order of include/tvbk.h
+    L.MODEL_G2D: ("tau", "I", "a", "b", "c", "d", "e", "f", "g", "alpha", "beta", "gamma"),
+    L.MODEL_MPR: ("tau", "I", "Delta", "J", "eta", "Gamma", "cr", "cv"),
+    L.MODEL_JR: ("A", "B", "a", "b", "v0", "nu_max", "r", "J", "a_1", "a_2", "a_3", "a_4", "p_min",
+                 "p_max", "mu"),
+}
+CFUNS = {"Linear": (L.CFUN_LINEAR, ("a", "b")),
+         "SigmoidalJansenRit": (L.CFUN_SIGMOID_JR, ("cmin", "cmax", "midpoint", "r", "a"))}
+SCHEMES = {"HeunDeterministic": L.HEUN_DET, "HeunStochastic": L.HEUN_STO,
+           "EulerDeterministic": L.EULER_DET, "EulerStochastic": L.EULER_STO}
+MONITORS = {"Raw": L.MON_RAW, "TemporalAverage": L.MON_TAVG, "Bold": L.MON_BOLD}
+PRECISIONS = {"fp32": L.F32, "float32": L.F32, "fp64": L.F64, "float64": L.F64}
+
+
+def _name(obj):
+    return type(obj).__name__
+
+
+def observables(model):
+    """Linear coefficients of the model's variables_of_interest over its
+    state variables (the observer of models/base.py:56-71)."""
+    svars = list(model.state_variables)
+    eye = np.eye(len(svars))
+    ns = {sv: eye[i] for i, sv in enumerate(svars)}
+    rows = []
+    for expr in model.variables_of_interest:
+        try:
+            v = eval(expr, {"__builtins__": {}}, dict(ns))
+        except Exception:
+            raise NotImplementedError("variable of interest %r is not a linear combination of state variables" % expr)
+        v = np.asarray(v, np.float64)
+        if v.shape != (len(svars),):
+            raise NotImplementedError("variable of interest %r is not linear" % expr)
+        rows.append(v)
+    return np.array(rows)
+
+
+def monitor_spec(mon, dt):
+    kind = MONITORS[_name(mon)]
+    voi = [int(v) for v in np.atleast_1d(mon.voi)]
+    if kind == L.MON_BOLD:
+        hrf = np.asarray(mon.hemodynamic_response_function, np.float64).reshape(-1)
+        taps = hrf[::-1].copy()                       # G in forward time order
+        kern = mon.hrf_kernel
+        if _name(kern) == "FirstOrderVolterra":
+            k1v0, sub = kern.parameters["k_1"] * kern.parameters["V_0"], True
+        else:
+            k1v0, sub = 1.0, False
+        return MonitorSpec(kind, mon.istep, voi, interim_istep=mon._interim_istep, taps=taps,
+                           k1_v0=k1v0, subtract_one=sub, period=mon.period, name="Bold")
+    return MonitorSpec(kind, mon.istep, voi, period=getattr(mon, "period", None), name=_name(mon))
+
+
+def _param_vector(sim, model_id, cfun_params, stochastic, N):
+    """Raw per-instance parameters (and spatial per-node values) of a sim."""
+    names = MODEL_PARAMS[model_id]
+    vals, spatial = [], {}
+    for k, nm in enumerate(names):
+        v = np.asarray(getattr(sim.model, nm), np.float64).reshape(-1)
+        if v.size == 1:
+            vals.append(float(v[0]))
+        elif v.size == N:
+            vals.append(float(v[0]))
+            spatial[k] = v
+        else:
+            raise NotImplementedError("parameter %s has shape %s" % (nm, v.shape))
+    for nm in cfun_params:
+        v = np.asarray(getattr(sim.coupling, nm), np.float64).reshape(-1)
+        if v.size != 1:
+            raise NotImplementedError("coupling parameter %s must be scalar" % nm)
+        vals.append(float(v[0]))
+    if stochastic:
+        nsv = len(sim.model.state_variables)
+        nsig = np.asarray(sim.integrator.noise.nsig, np.float64)
+        base = len(vals)
+        if nsig.size == 1:
+            vals += [float(nsig.reshape(-1)[0])] * nsv
+        elif nsig.shape in ((nsv, 1, 1), (nsv,)):
+            vals += [float(x) for x in nsig.reshape(-1)]
+        elif nsig.shape in ((1, N, 1), (N,)):
+            vals += [float(nsig.reshape(-1)[0])] * nsv
+            for v in range(nsv):
+                spatial[base + v] = nsig.reshape(-1)
+        elif nsig.shape in ((nsv, N, 1), (nsv, N)):
+            n2 = nsig.reshape((nsv, N))
+            vals += [float(x) for x in n2[:, 0]]
+            for v in range(nsv):
+                spatial[base + v] = n2[v]
+        else:
+            raise NotImplementedError("noise nsig shape %s" % (nsig.shape,))
+    return np.array(vals, np.float64), spatial
+
+
+class HipBackend:
+    """Runs configured simulators with the HIP step kernels."""
+
+    # ---------------------------------------------------------------- checks
+    def check_compatibility(self, sim):
+        def need(cond, what):
+            if not cond:
+                raise NotImplementedError("Unsupported simulator component. " + what)
+        need(getattr(sim, "surface", None) is None, "Surface simulation not supported.")
+        need(getattr(sim, "stimulus", None) is None, "Stimulus not supported.")
+        need(_name(sim.model) in MODELS, "Given: %s\nExpected one of: %s" % (_name(sim.model), tuple(MODELS)))
+        need(getattr(sim.model, "number_of_modes", 1) == 1, "Multi-mode models not supported.")
+        need(_name(sim.coupling) in CFUNS, "Given: %s\nExpected one of: %s" % (_name(sim.coupling), tuple(CFUNS)))
+        need(not (_name(sim.coupling) == "SigmoidalJansenRit" and _name(sim.model) != "JansenRit"),
+             "SigmoidalJansenRit coupling needs a model with two coupling variables (JansenRit).")
+        need(_name(sim.integrator) in SCHEMES, "Given: %s\nExpected one of: %s" % (_name(sim.integrator), tuple(SCHEMES)))
+        if _name(sim.integrator).endswith("Stochastic"):
+            need(_name(sim.integrator.noise) == "Additive", "Only Additive noise is supported.")
+            need(float(getattr(sim.integrator.noise, "ntau", 0.0)) == 0.0, "Coloured noise not supported.")
+        mons = sim.monitors if isinstance(sim.monitors, (list, tuple)) else [sim.monitors]
+        need(1 <= len(mons) <= L.MAX_MON, "Configure with 1 to %d monitors." % L.MAX_MON)
+        for m in mons:
+            need(_name(m) in MONITORS, "Given: %s\nExpected one of: %s" % (_name(m), tuple(MONITORS)))
+        observables(sim.model)
+
+    # ------------------------------------------------------------ plan build
+    def _components(self, sim):
+        model_id = MODELS[_name(sim.model)]
+        cfun_id, cfun_params = CFUNS[_name(sim.coupling)]
+        scheme = SCHEMES[_name(sim.integrator)]
+        stochastic = scheme in (L.HEUN_STO, L.EULER_STO)
+        mons = sim.monitors if isinstance(sim.monitors, (list, tuple)) else [sim.monitors]
+        specs = [monitor_spec(m, sim.integrator.dt) for m in mons]
+        return model_id, cfun_id, cfun_params, scheme, stochastic, specs
+
+    def _initial(self, sim, precision, cfun_id, csr):
+        """History channels (H, nch, N) and state from the configured sim."""
+        use64 = precision == L.F64 and getattr(sim.history, "initial_history", None) is not None
+        if use64:
+            hist = np.asarray(sim.history.initial_history, np.float64)[..., 0]       # (H, nsv, N)
+            cv = hist[:, np.asarray(sim.model.cvar)]
+        else:
+            cv = np.asarray(sim.history.buffer)[..., 0]                              # (H, ncvar, N) f32
+        if cfun_id == L.CFUN_SIGMOID_JR:
+            c = sim.coupling
+            diff = (cv[:, 0] - cv[:, 1])                       # float32 for the reference buffer
+            pre = c.cmin + (c.cmax - c.cmin) / (1.0 + np.exp(c.r * (c.midpoint - diff)))
+            cv = pre[:, np.newaxis, :].astype(np.float64)
+        state = np.asarray(sim.current_state, np.float64)[..., 0]
+        return np.ascontiguousarray(cv), state
+
+    def build_plan(self, sim, precision="fp32", noise="legacy", max_steps=65536, kernel="auto",
+                   tile=0, waves=0, seed=None, stream=None):
+        """A one-instance Plan from a configured simulator, with its history
+        and state uploaded."""
+        self.check_compatibility(sim)
+        prec = PRECISIONS[precision]
+        model_id, cfun_id, cfun_params, scheme, stochastic, specs = self._components(sim)
+        N = sim.connectivity.number_of_regions
+        csr = CSR(sim.connectivity.weights, sim.connectivity.idelays, sim.connectivity.horizon)
+        params, spatial = _param_vector(sim, model_id, cfun_params, stochastic, N)
+        node_params, mask = None, 0
+        if spatial:
+            node_params = np.repeat(params[:, None], N, axis=1)
+            for k, v in spatial.items():
+                node_params[k] = v
+                mask |= 1 << k
+        if not stochastic:
+            noise_id = L.NOISE_NONE
+        elif noise == "philox":
+            noise_id = L.NOISE_PHILOX
+        else:
+            noise_id = L.NOISE_INJECTED
+        if seed is None:
+            seed = int(getattr(getattr(sim.integrator, "noise", None), "noise_seed", 42) or 0)
+        kern = {"auto": L.KERNEL_AUTO, "batch": L.KERNEL_BATCH_LANE, "edge": L.KERNEL_EDGE_LANE}[kernel]
+        plan = Plan(model=model_id, cfun=cfun_id, scheme=scheme, precision=prec, noise=noise_id,
+                    conns=[csr], params=params[None, :], dt=sim.integrator.dt,
+                    obs_coef=observables(sim.model), monitors=specs, max_steps=max_steps,
+                    node_params=node_params, node_param_mask=mask, seed=seed, kernel=kern,
+                    tile=tile, waves=waves, stream=stream)
+        hist, state = self._initial(sim, prec, cfun_id, csr)
+        plan.set_history(0, hist, state, sim.current_step)
+        return plan
+
+    # --------------------------------------------------------------- running
+    def run_sim(self, sim, nstep=None, simulation_length=None, precision="fp32", noise="legacy",
+                chunk=65536, kernel="auto", write_back=True, **_ignored):
+        """Run a configured simulator for nstep steps (or simulation_length ms).
+
+        noise: "legacy" draws the standard normals from the simulator's own
+        RandomState (noise.py:195-198) in the reference's order and injects
+        them; an array (nstep, nsvar, N) injects those values; "philox" uses
+        the device generator.  Returns [(t, x), ...] like Simulator.run."""
+        if nstep is None:
+            if simulation_length is None:
+                simulation_length = sim.simulation_length
+            nstep = int(np.ceil(simulation_length / sim.integrator.dt))   # simulator.py:351
+        nstep = int(nstep)
+        chunk = max(1, min(chunk, nstep))
+        plan = self.build_plan(sim, precision=precision, noise="philox" if noise == "philox" else "inject",
+                               max_steps=chunk, kernel=kernel)
+        try:
+            N = sim.connectivity.number_of_regions
+            nsv = len(sim.model.state_variables)
+            stochastic = plan.noise == L.NOISE_INJECTED
+            ts = [[] for _ in plan.monitors]
+            xs = [[] for _ in plan.monitors]
+            done = 0
+            while done < nstep:
+                n = min(chunk, nstep - done)
+                if stochastic:
+                    if isinstance(noise, np.ndarray):
+                        z = noise[done:done + n]
+                    else:
+                        z = sim.integrator.noise.random_stream.normal(size=(n, nsv, N))
+                    plan.set_noise(np.ascontiguousarray(np.asarray(z, np.float64).reshape(n, nsv, N, 1)))
+                plan.run(n)
+                for m in range(len(plan.monitors)):
+                    (t, x), = plan.read(m)
+                    ts[m].append(t)
+                    xs[m].append(np.asarray(x, np.float64))
+                done += n
+            flags = plan.flags()
+            if write_back:
+                hist, state, step = plan.export(0, np.float32)
+                sim.current_state = state[..., np.newaxis].copy()
+                sim.current_step = step
+                if _name(sim.coupling) != "SigmoidalJansenRit":
+                    sim.history.buffer = hist[..., np.newaxis].copy()
+            self.last_flags = flags
+            self.last_plan_shape = (plan.kernel, plan.tile, plan.waves, plan.lds_bytes)
+        finally:
+            plan.close()
+        out = []
+        for m in range(len(ts)):
+            t = np.concatenate(ts[m]) if ts[m] else np.zeros(0)
+            x = np.concatenate(xs[m], axis=0) if xs[m] else np.zeros((0,))
+            out.append((t, x[..., np.newaxis] if x.ndim == 3 else x))
+        return out

@@ -165,7 +165,8 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs a) {
         pp = pn;
       }
       if (a.noise == NOISE_INJECTED) {
-        const double* z = a.z + td.z_off + k * td.z_sk + (int64_t)i * td.z_si + t;
+        // padded lanes (t >= n_inst) read lane 0's noise: never past the array
+        const double* z = a.z + td.z_off + k * td.z_sk + (int64_t)i * td.z_si + (live ? t : 0);
 #pragma unroll
         for (int v = 0; v < NSV; ++v) xi[v] = (R)((a.sqrt_dt * z[v * td.z_ssv]) * (double)gf[v]);
       } else if (a.noise == NOISE_PHILOX) {

@@ -164,6 +164,7 @@ __device__ inline R reduceat_row(int n, F& f) {
   if (n <= 0) return R(0);
   R first = f(0);
   if (n == 1) return first;
+  if (n - 1 <= 128) return first + pw_leaf<R>(1, n - 1, f);
   return first + pw_sum<3, R>(1, n - 1, f);
 }
 

@@ -42,14 +42,33 @@ __device__ inline void copy_bytes(unsigned char* dst, const unsigned char* src, 
   for (int64_t k = threadIdx.x; k < n16; k += blockDim.x) d[k] = s[k];
 }
 
+// Tile, monitor and observable configuration staged in LDS once per launch:
+// read from LDS inside the step loop instead of being held in registers.
+struct KCfg {
+  TileDesc td;
+  MonArgs mon[TVBK_KMAX_MON];
+  int32_t obs_sv[TVBK_KMAX_OBS];
+  double obs_coef[TVBK_KMAX_OBS][8];
+};
+
 template <class R, class M, int CF>
 __global__ void __launch_bounds__(1024) k_step(StepArgs a) {
   using C = Cfun<M, CF>;
   constexpr int NSV = M::NSV, NCH = C::NCH, NCT = C::NCT, NDP = M::NDP, NCP = C::NCP;
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ KCfg kc;
 
-  const TileDesc td = a.tiles[blockIdx.x];
-  if (td.n_steps <= 0) return;
+  if (a.tiles[blockIdx.x].n_steps <= 0) return;
+  if (threadIdx.x == 0) {
+    kc.td = a.tiles[blockIdx.x];
+    for (int m = 0; m < TVBK_KMAX_MON; ++m) kc.mon[m] = a.mon[m];
+    for (int o = 0; o < TVBK_KMAX_OBS; ++o) {
+      kc.obs_sv[o] = a.obs_sv[o];
+      for (int u = 0; u < 8; ++u) kc.obs_coef[o][u] = a.obs_coef[o][u];
+    }
+  }
+  __syncthreads();
+  const TileDesc& td = kc.td;
   const ConnDesc cd = a.conns[td.conn];
   const int T = a.T;
   const int N = cd.n_node, H = cd.horizon;
@@ -104,23 +123,23 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs a) {
   // ---- step counters (wave-uniform) --------------------------------------
   int64_t s = td.step0;
   int smod = (int)((s - 1) % H);          // (s-1) % H ; s >= 1
-  int mcnt[TVBK_KMAX_MON], icnt[TVBK_KMAX_MON];
-  int64_t msample[TVBK_KMAX_MON], mpush[TVBK_KMAX_MON];
-#pragma unroll
-  for (int m = 0; m < TVBK_KMAX_MON; ++m) {
-    if (m < a.n_mon) {
-      mcnt[m] = (int)(s % a.mon[m].istep);
-      icnt[m] = a.mon[m].kind == MON_BOLD ? (int)(s % a.mon[m].ii) : 0;
-      msample[m] = 0;
-      mpush[m] = a.mon[m].kind == MON_BOLD ? s / a.mon[m].ii : 0;
+  if (threadIdx.x == 0) {
+    for (int m = 0; m < a.n_mon; ++m) {
+      MonArgs& mo = kc.mon[m];
+      mo.r0 = (int)(s % mo.istep);
+      mo.qe0 = s / mo.istep;
+      mo.rb0 = (int)(s % mo.ii);
+      mo.qb0 = s / mo.ii;
     }
   }
+  __syncthreads();
   int bad = 0;
   const int P = N * NCH;
   const int NG = 64 / T;        // lane groups per wave
   const int g = lane / T;
+  const int nsteps = (int)td.n_steps;
 
-  for (int64_t k = 0; k < td.n_steps; ++k, ++s) {
+  for (int k = 0; k < nsteps; ++k, ++s) {
     const int buf = (int)(k & 1);
     R* cbuf = s_c + (int64_t)buf * N * NCT * T;
     // ================= phase A: coupling of step s ========================
@@ -189,25 +208,29 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs a) {
       for (int ch = 0; ch < NCH; ++ch) wr[ch * T] = C::template channel<R>(x, ch, cp);
 
       // ---------------- monitors (monitors.py:160-190, :363-391, :834-962)
-#pragma unroll
-      for (int m = 0; m < TVBK_KMAX_MON; ++m) {
-        if (m >= a.n_mon) break;
-        const MonArgs& mo = a.mon[m];
+      for (int m = 0; m < a.n_mon; ++m) {
+        const MonArgs& mo = kc.mon[m];
         const int nv = mo.nvoi;
-        const bool emit = (mcnt[m] == 0);
+        // emission when s % istep == 0; sample index within this call
+        const int ek = mo.r0 + k;
+        const bool emit = (ek % mo.istep) == 0;
+        const int64_t msample = ek / mo.istep - (mo.r0 > 0 ? 1 : 0);
         if (mo.kind == MON_RAW || mo.kind == MON_TAVG) {
-          R* out = reinterpret_cast<R*>(a.out[m]) + td.out_off[m] + msample[m] * td.out_sj[m] +
+          R* out = reinterpret_cast<R*>(a.out[m]) + td.out_off[m] + msample * td.out_sj[m] +
                    (int64_t)i * td.out_si[m] + t;
           R* ts = reinterpret_cast<R*>(smem + mo.lds_off);
           for (int v = 0; v < nv; ++v) {
             const int o = mo.voi[v];
             R val;
-            if (a.obs_sv[o] >= 0) {
-              val = x[a.obs_sv[o]];
+            const int osv = kc.obs_sv[o];
+            if (osv >= 0) {
+              val = x[0];
+#pragma unroll
+              for (int u = 1; u < NSV; ++u) val = (u == osv) ? x[u] : val;
             } else {
               val = R(0);
 #pragma unroll
-              for (int u = 0; u < NSV; ++u) val = val + (R)a.obs_coef[o][u] * x[u];
+              for (int u = 0; u < NSV; ++u) val = val + (R)kc.obs_coef[o][u] * x[u];
             }
             if (mo.kind == MON_RAW) {
               if (live) out[v * td.out_sv[m]] = val;
@@ -234,27 +257,31 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs a) {
           }
         } else {  // Bold: interim mean every ii steps pushed into FIR accumulators
           double* bi = reinterpret_cast<double*>(smem + mo.lds_off);
-          const bool push = (icnt[m] == 0);
+          const int pk = mo.rb0 + k;
+          const bool push = (pk % mo.ii) == 0;
+          const int64_t mp = mo.qb0 + pk / mo.ii;        // interim sample index s / ii
           double* acc = a.bacc + td.bacc_off[m];
           for (int v = 0; v < nv; ++v) {
             const int o = mo.voi[v];
             R val;
-            if (a.obs_sv[o] >= 0) {
-              val = x[a.obs_sv[o]];
+            const int osv = kc.obs_sv[o];
+            if (osv >= 0) {
+              val = x[0];
+#pragma unroll
+              for (int u = 1; u < NSV; ++u) val = (u == osv) ? x[u] : val;
             } else {
               val = R(0);
 #pragma unroll
-              for (int u = 0; u < NSV; ++u) val = val + (R)a.obs_coef[o][u] * x[u];
+              for (int u = 0; u < NSV; ++u) val = val + (R)kc.obs_coef[o][u] * x[u];
             }
             double* bsum = bi + (i * nv + v) * T + t;
             *bsum = *bsum + (double)val;
             if (push) {
               const double mean = *bsum / (double)mo.ii;
               *bsum = 0.0;
-              // sample m' = mpush contributes to emissions kk with
-              // kk*istep >= m'*ii and kk*istep < (m'+S)*ii; weight by age
-              // a = floor(kk*istep/ii) - m': a == 0 -> G[S-1], else G[a-1]
-              const int64_t mp = mpush[m];
+              // sample mp contributes to emissions kk with kk*istep >= mp*ii
+              // and kk*istep < (mp+S)*ii; weight by age a = floor(kk*istep/ii) - mp:
+              // a == 0 -> G[S-1], else G[a-1]
               const int64_t klo = (mp * mo.ii + mo.istep - 1) / mo.istep;
               const int64_t khi = ((mp + mo.S) * mo.ii - 1) / mo.istep;
               for (int64_t kk = klo; kk <= khi; ++kk) {
@@ -265,9 +292,9 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs a) {
               }
             }
             if (emit) {
-              const int64_t kk = s / mo.istep;
+              const int64_t kk = mo.qe0 + ek / mo.istep;   // s / istep
               double* ap = acc + (((kk % mo.Q) * nv + v) * (int64_t)N + i) * T + t;
-              double* out = reinterpret_cast<double*>(a.out[m]) + td.out_off[m] + msample[m] * td.out_sj[m] +
+              double* out = reinterpret_cast<double*>(a.out[m]) + td.out_off[m] + msample * td.out_sj[m] +
                             (int64_t)v * td.out_sv[m] + (int64_t)i * td.out_si[m] + t;
               const double bold = mo.sub1 ? (*ap - 1.0) * mo.k1v0 : *ap;
               if (live) *out = bold;
@@ -277,18 +304,7 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs a) {
         }
       }
     }
-    // ---- advance counters
     smod = wslot;
-#pragma unroll
-    for (int m = 0; m < TVBK_KMAX_MON; ++m) {
-      if (m >= a.n_mon) break;
-      if (mcnt[m] == 0) msample[m] += 1;
-      mcnt[m] = (mcnt[m] + 1 == a.mon[m].istep) ? 0 : mcnt[m] + 1;
-      if (a.mon[m].kind == MON_BOLD) {
-        if (icnt[m] == 0) mpush[m] += 1;
-        icnt[m] = (icnt[m] + 1 == a.mon[m].ii) ? 0 : icnt[m] + 1;
-      }
-    }
     if (a.zero_delay) __syncthreads();
   }
   // note: msample counts emissions of this call; the host derives the same

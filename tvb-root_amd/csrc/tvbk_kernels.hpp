@@ -29,6 +29,10 @@ struct MonArgs {
   int32_t voi[TVBK_KMAX_OBS];
   double k1v0;
   const double* taps;
+  // per-tile phase of the first step s0 of a call, filled on the device:
+  // r0 = s0 % istep, qe0 = s0 / istep, rb0 = s0 % ii, qb0 = s0 / ii
+  int32_t r0, rb0;
+  int64_t qe0, qb0;
 };
 
 struct StepArgs {

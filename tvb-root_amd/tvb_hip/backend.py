@@ -211,7 +211,8 @@ class HipBackend:
             nstep = int(np.ceil(simulation_length / sim.integrator.dt))   # simulator.py:351
         nstep = int(nstep)
         chunk = max(1, min(chunk, nstep))
-        plan = self.build_plan(sim, precision=precision, noise="philox" if noise == "philox" else "inject",
+        philox_noise = isinstance(noise, str) and noise == "philox"
+        plan = self.build_plan(sim, precision=precision, noise="philox" if philox_noise else "inject",
                                max_steps=chunk, kernel=kernel)
         try:
             N = sim.connectivity.number_of_regions
@@ -251,3 +252,133 @@ class HipBackend:
             x = np.concatenate(xs[m], axis=0) if xs[m] else np.zeros((0,))
             out.append((t, x[..., np.newaxis] if x.ndim == 3 else x))
         return out
+
+    # ---------------------------------------------------------------- batches
+    def build_batch_plan(self, sims, precision="fp32", noise="inject", max_steps=65536, kernel="auto",
+                         tile=0, waves=0, seed=0, streams=None, stream=None):
+        """One Plan over several configured simulators of the same model,
+        coupling, integrator and monitors.  Simulators whose connectivity has
+        identical weights and delays share one connectome (and can use the
+        batch-lane kernel); others get their own.  Each instance starts from
+        its simulator's history and current state."""
+        sims = list(sims)
+        if not sims:
+            raise ValueError("no simulators")
+        for s in sims:
+            self.check_compatibility(s)
+        prec = PRECISIONS[precision]
+        model_id, cfun_id, cfun_params, scheme, stochastic, specs = self._components(sims[0])
+        dt = sims[0].integrator.dt
+        key0 = (model_id, cfun_id, scheme, [(m.kind, m.istep, tuple(m.voi)) for m in specs])
+        conns, inst_conn, sig = [], [], []
+        params, spatial_any = [], False
+        for s in sims:
+            mi, ci, cpn, sc, st, sp = self._components(s)
+            if (mi, ci, sc, [(m.kind, m.istep, tuple(m.voi)) for m in sp]) != key0 or s.integrator.dt != dt:
+                raise ValueError("all simulators of a batch need the same components, monitors and dt")
+            w, d = s.connectivity.weights, s.connectivity.idelays
+            for k, (w2, d2) in enumerate(sig):
+                if w2.shape == w.shape and np.array_equal(w2, w) and np.array_equal(d2, d):
+                    inst_conn.append(k)
+                    break
+            else:
+                sig.append((w, d))
+                conns.append(CSR(w, d, s.connectivity.horizon))
+                inst_conn.append(len(conns) - 1)
+            pv, spatial = _param_vector(s, model_id, cfun_params, stochastic, s.connectivity.number_of_regions)
+            spatial_any |= bool(spatial)
+            params.append(pv)
+        if spatial_any:
+            raise NotImplementedError("spatial (per-node) parameters are supported for single simulators only")
+        noise_id = L.NOISE_NONE if not stochastic else (L.NOISE_PHILOX if noise == "philox" else L.NOISE_INJECTED)
+        kern = {"auto": L.KERNEL_AUTO, "batch": L.KERNEL_BATCH_LANE, "edge": L.KERNEL_EDGE_LANE}[kernel]
+        plan = Plan(model=model_id, cfun=cfun_id, scheme=scheme, precision=prec, noise=noise_id, conns=conns,
+                    params=np.array(params), dt=dt, obs_coef=observables(sims[0].model), monitors=specs,
+                    max_steps=max_steps, inst_conn=np.array(inst_conn, np.int32) if len(conns) > 1 else None,
+                    seed=seed, inst_stream=streams, kernel=kern, tile=tile, waves=waves, stream=stream)
+        for b, s in enumerate(sims):
+            hist, state = self._initial(s, prec, cfun_id, conns[inst_conn[b]])
+            plan.set_history(b, hist, state, s.current_step)
+        return plan
+
+    def run_batch(self, sims, nstep, precision="fp32", noise="inject", steps=None, kernel="auto", tile=0,
+                  seed=0, streams=None):
+        """Run several simulators in one launch.  ``nstep``: steps of the
+        longest; ``steps``: per-simulator step counts (uneven durations).
+        ``noise``: "philox", or an array per simulator of standard normals
+        (n, nsvar, N) to inject.  Returns one ``[(t, x), ...]`` per simulator."""
+        sims = list(sims)
+        inject = not (isinstance(noise, str) and noise == "philox")
+        plan = self.build_batch_plan(sims, precision=precision, noise="inject" if inject else "philox",
+                                     max_steps=nstep, kernel=kernel, tile=tile, seed=seed, streams=streams)
+        try:
+            if steps is not None:
+                plan.set_steps(np.asarray(steps, np.int64))
+            if plan.noise == L.NOISE_INJECTED:
+                nmax = max(c.n_node for c in plan.conns)
+                nsv = len(sims[0].model.state_variables)
+                z = np.zeros((nstep, nsv, nmax, len(sims)))
+                for b, s in enumerate(sims):
+                    zb = noise[b] if not isinstance(noise, str) else \
+                        s.integrator.noise.random_stream.normal(size=(nstep, nsv, s.connectivity.number_of_regions))
+                    zb = np.asarray(zb, np.float64)
+                    z[:zb.shape[0], :, :zb.shape[2], b] = zb
+                plan.set_noise(z)
+            plan.run(nstep)
+            out = [[] for _ in sims]
+            for m in range(len(plan.monitors)):
+                for b, (t, x) in enumerate(plan.read(m)):
+                    out[b].append((t, np.asarray(x, np.float64)[..., np.newaxis]))
+            self.last_flags = plan.flags()
+            self.last_plan_shape = (plan.kernel, plan.tile, plan.waves, plan.lds_bytes)
+        finally:
+            plan.close()
+        return out
+
+    def build_sweep_plan(self, sim, overrides, precision="fp32", noise="philox", max_steps=65536,
+                         kernel="auto", tile=0, waves=0, seed=0, init="sim", init_seed=1, stream=None):
+        """A Plan of B instances of one simulator with per-instance parameter
+        overrides, e.g. {"coupling.a": G, "noise.nsig": sigma, "model.eta": eta}
+        (arrays of length B; "noise.nsig" sets every state variable).
+        init="sim" starts every instance from the simulator's history;
+        init="uniform" draws each instance's history on the device uniformly
+        in the model's state_variable_range (seeded by init_seed)."""
+        self.check_compatibility(sim)
+        prec = PRECISIONS[precision]
+        model_id, cfun_id, cfun_params, scheme, stochastic, specs = self._components(sim)
+        N = sim.connectivity.number_of_regions
+        base, spatial = _param_vector(sim, model_id, cfun_params, stochastic, N)
+        if spatial:
+            raise NotImplementedError("spatial parameters in sweeps are not supported")
+        B = len(next(iter(overrides.values())))
+        params = np.repeat(base[None, :], B, axis=0)
+        names = {("model." + n): k for k, n in enumerate(MODEL_PARAMS[model_id])}
+        names.update({("coupling." + n): len(MODEL_PARAMS[model_id]) + k for k, n in enumerate(cfun_params)})
+        nsv = len(sim.model.state_variables)
+        for key, vals in overrides.items():
+            vals = np.asarray(vals, np.float64)
+            if vals.shape != (B,):
+                raise ValueError("override %s must have shape (%d,)" % (key, B))
+            if key == "noise.nsig":
+                if not stochastic:
+                    raise ValueError("noise.nsig needs a stochastic integrator")
+                params[:, -nsv:] = vals[:, None]
+            elif key in names:
+                params[:, names[key]] = vals
+            else:
+                raise KeyError("unknown sweep parameter %s" % key)
+        noise_id = L.NOISE_NONE if not stochastic else (L.NOISE_PHILOX if noise == "philox" else L.NOISE_INJECTED)
+        kern = {"auto": L.KERNEL_AUTO, "batch": L.KERNEL_BATCH_LANE, "edge": L.KERNEL_EDGE_LANE}[kernel]
+        csr = CSR(sim.connectivity.weights, sim.connectivity.idelays, sim.connectivity.horizon)
+        plan = Plan(model=model_id, cfun=cfun_id, scheme=scheme, precision=prec, noise=noise_id, conns=[csr],
+                    params=params, dt=sim.integrator.dt, obs_coef=observables(sim.model), monitors=specs,
+                    max_steps=max_steps, seed=seed, kernel=kern, tile=tile, waves=waves, stream=stream)
+        if init == "uniform":
+            svr = sim.model.state_variable_range
+            lo = [float(svr[sv][0]) for sv in sim.model.state_variables]
+            hi = [float(svr[sv][1]) for sv in sim.model.state_variables]
+            plan.init_uniform(init_seed, lo, hi)
+        else:
+            hist, state = self._initial(sim, prec, cfun_id, csr)
+            plan.set_history(-1, hist, state, sim.current_step)
+        return plan

@@ -159,6 +159,41 @@ __device__ inline R pw_sum(int s, int n, F& f) {
   }
 }
 
+// Same order for rows of at most RMAX entries, with every load of the row
+// issued before the first add (the row is staged in registers): one memory
+// round trip per row instead of one per block of eight.
+template <int RMAX, class R, class F>
+__device__ inline R reduceat_row_regs(int n, F& f) {
+  R v[RMAX];
+#pragma unroll
+  for (int e = 0; e < RMAX; ++e) v[e] = (e < n) ? f(e) : R(0);
+  if (n <= 1) return n == 1 ? v[0] : R(0);
+  const int m = n - 1;                       // pairwise part v[1..n-1]
+  R res;
+  if (m < 8) {
+    res = R(0);
+#pragma unroll
+    for (int i = 0; i < 7; ++i)
+      if (i < m) res += v[1 + i];
+  } else {
+    const int nfull = m - (m & 7);
+    R r[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) r[k] = v[1 + k];
+#pragma unroll
+    for (int b = 8; b + 8 <= RMAX - 1; b += 8)
+      if (b < nfull) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) r[k] += v[1 + b + k];
+      }
+    res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+#pragma unroll
+    for (int i = 8; i < RMAX - 1; ++i)
+      if (i >= nfull && i < m) res += v[1 + i];
+  }
+  return v[0] + res;
+}
+
 template <class R, class F>
 __device__ inline R reduceat_row(int n, F& f) {
   if (n <= 0) return R(0);

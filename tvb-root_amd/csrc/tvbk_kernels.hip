@@ -156,7 +156,8 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs a) {
           sl += (sl < 0) ? H : 0;
           return e_w[e0 + e] * rb[(sl * N + j) * rstride];
         };
-        R acc = reduceat_row<R>(n, prod);
+        constexpr int RMAX = sizeof(R) == 4 ? 32 : 24;
+        R acc = (n <= RMAX) ? reduceat_row_regs<RMAX, R>(n, prod) : reduceat_row<R>(n, prod);
         if (CF == CF_SIGMOID_JR) {
           cbuf[(i * NCT) * T + t] = C::post(acc, cp);
         } else {
@@ -173,15 +174,15 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs a) {
       for (int v = 0; v < NSV; ++v) x[v] = s_x[(i * NSV + v) * T + t];
 #pragma unroll
       for (int v = 0; v < NCT; ++v) c[v] = cbuf[(i * NCT + v) * T + t];
-      const R* pp = pm;
-      R pn[NDP > 0 ? NDP : 1];
+      R pp[NDP > 0 ? NDP : 1];
+#pragma unroll
+      for (int v = 0; v < NDP; ++v) pp[v] = pm[v];
       if (a.spatial) {
         const R* pr = reinterpret_cast<const R*>(a.params) + (int64_t)inst * spatial_row + (int64_t)i * a.prow;
 #pragma unroll
-        for (int v = 0; v < NDP; ++v) pn[v] = pr[v];
+        for (int v = 0; v < NDP; ++v) pp[v] = pr[v];
 #pragma unroll
         for (int v = 0; v < NSV; ++v) gf[v] = (a.noise != NOISE_NONE) ? pr[NDP + NCP + v] : R(0);
-        pp = pn;
       }
       if (a.noise == NOISE_INJECTED) {
         // padded lanes (t >= n_inst) read lane 0's noise: never past the array

@@ -158,6 +158,74 @@ def tavg_error_sample(w, L, d, G, sig, n_steps=4000, B=32, check=(0, 17)):
     return max(errs)
 
 
+def c3_plan(n_inst=256, duration_scale=0.02, seed=303, stream=None):
+    """Uneven batch: JansenRit + SigmoidalJansenRit, EulerStochastic dt 0.1,
+    node counts from {68, 96, 192, 400, 998}, a distinct seeded connectome
+    per instance (density U[0.1, 0.3], lengths U[10, 240] mm at 3 mm/ms ->
+    delays up to 800 steps), durations U[2, 30] s x duration_scale,
+    per-instance a, b ~ U[0.9, 1.1] x defaults, TemporalAverage(2 ms) + Bold."""
+    from tvb_hip import _lib as L
+    from tvb_hip.connectome import idelays, synthetic
+    from tvb_hip.plan import CSR, MonitorSpec, Plan
+    from tvb_hip.sim import iround
+    rng = np.random.default_rng(seed)
+    dt = 0.1
+    conns, steps, params = [], [], []
+    base = [3.25, 22.0, 0.1, 0.05, 5.52, 0.0025, 0.56, 135.0, 1.0, 0.8, 0.25, 0.25, 0.12, 0.32, 0.22]
+    for b in range(n_inst):
+        n = int(rng.choice([68, 96, 192, 400, 998]))
+        w, Lm = synthetic(n, float(rng.uniform(0.1, 0.3)), 10.0, 240.0, int(rng.integers(1 << 30)))
+        d, H = idelays(Lm, 3.0, dt)
+        conns.append(CSR(w, d, H))
+        steps.append(int(np.ceil(rng.uniform(2000.0, 30000.0) * duration_scale / dt)))
+        p = list(base)
+        p[2] *= rng.uniform(0.9, 1.1)
+        p[3] *= rng.uniform(0.9, 1.1)
+        params.append(p + [0.0, 0.005, 6.0, 0.56, 1.0] + [0.0, 0.0, 0.0, 1e-3, 1e-3, 1e-3])
+    mons = [MonitorSpec(L.MON_TAVG, iround(2.0 / dt), [0, 1, 2, 3], name="TemporalAverage"),
+            make_bold_spec(dt, 2000.0)]
+    mons[1].voi = [0, 1, 2, 3]
+    plan = Plan(model=L.MODEL_JR, cfun=L.CFUN_SIGMOID_JR, scheme=L.EULER_STO, precision=L.F32,
+                noise=L.NOISE_PHILOX, conns=conns, params=np.array(params), dt=dt, obs_coef=np.eye(6)[:4],
+                monitors=mons, max_steps=max(steps), inst_conn=np.arange(n_inst, dtype=np.int32), seed=seed,
+                stream=stream)
+    plan.set_steps(np.array(steps, np.int64))
+    lo = [-1.0, -500.0, -50.0, -6.0, -20.0, -500.0]
+    plan.init_uniform(seed + 1, [x * 0.01 for x in lo], [-x * 0.01 for x in lo])
+    rs = float(sum(c.n_node * s for c, s in zip(conns, steps)))
+    return plan, max(steps), rs, dict(instances=n_inst, duration_scale=duration_scale,
+                                      total_nnz=int(sum(c.nnz for c in conns)))
+
+
+def run_c3(stream=None, n_inst=256, duration_scale=0.02):
+    plan, nmax, rs, info = c3_plan(n_inst, duration_scale, stream=stream)
+    t0 = time.perf_counter()
+    plan.run(nmax)
+    plan.sync()
+    el = time.perf_counter() - t0
+    flags = plan.flags()
+    plan.close()
+    info.update(value=rs / el, unit="region-steps/s", seconds=el, nonfinite=int((flags != 0).sum()))
+    return info
+
+
+def run_c4(n_requests=24, rate_hz=40.0, duration_scale=0.01, inst_range=(1, 64), seed=404):
+    """Serving: Poisson stream of sweep requests (G2D or MPR, HeunStochastic,
+    N log-uniform in [68, 998], instances in inst_range, durations
+    U[5, 60] s x duration_scale), admitted by device footprint."""
+    import torch
+    from tvb_hip.serve import Server, synthetic_requests
+    reqs = synthetic_requests(n_requests, rate_hz, seed=seed, inst_range=inst_range,
+                              duration_scale=duration_scale)
+    streams = [torch.cuda.Stream() for _ in range(4)]
+    free, total = torch.cuda.mem_get_info()
+    srv = Server(hbm_budget=int(0.5 * free), streams=[s.cuda_stream for s in streams], chunk_steps=2000,
+                 keep_outputs=True)
+    stats = srv.run(reqs)
+    stats.update(duration_scale=duration_scale, rate_hz=rate_hz, inst_range=list(inst_range))
+    return stats
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +237,7 @@ def main():
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=40000)
     ap.add_argument("--no-extras", action="store_true", help="skip CPU baseline, FP64 single path and error sample")
+    ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4"))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: every instance of the sweep is independent and the "
@@ -180,6 +249,14 @@ def main():
     from tvb_hip import _lib as L
     assert torch.cuda.is_available(), "no GPU"
     stream = torch.cuda.current_stream().cuda_stream
+    if args.workload == "c3":
+        print(json.dumps({"metric": "region-steps/s (uneven JansenRit batch)", "workload": "c3",
+                          **run_c3(stream)}), flush=True)
+        return
+    if args.workload == "c4":
+        print(json.dumps({"metric": "p50/p99 request latency (serving)", "workload": "c4", **run_c4()}),
+              flush=True)
+        return
     B = args.instances
     dt = 0.05
     w, Lmat, d, H, G, sig = c2_setup(B)
@@ -239,6 +316,10 @@ def main():
                                         "kernel": list(shape64)}
         line["batched_fp32_over_fp64_single"] = value / v64
         line["tavg_max_rel_err_vs_fp64_oracle"] = tavg_error_sample(w, Lmat, d, G, sig)
+        line["serving"] = run_c4()
+        line["p50_latency_s"] = line["serving"]["p50_latency_s"]
+        line["p99_latency_s"] = line["serving"]["p99_latency_s"]
+        line["uneven_batch"] = run_c3(stream)
         cpu_rate, cpu_el = cpu_baseline(w, Lmat, 1.0, 1e-3, args.cpu_steps)
         line["cpu_baseline"] = {"value": cpu_rate, "unit": "region-steps/s", "cores": 1, "kind": "port",
                                 "sample": "1 instance x %d steps (%.1f s simulated) of the c2 workload, "

@@ -162,6 +162,15 @@ int tvbk_plan_create(const tvbk_plan_desc *desc, void *stream, tvbk_plan **out);
 
 int tvbk_plan_destroy(tvbk_plan *plan);
 
+/* Device bytes tvbk_plan_create would allocate for `desc` (no allocation);
+ * the admission budget of the request loop.  The reference's nearest
+ * counterpart is Simulator._guesstimate_memory_requirement /
+ * _census_memory_requirement (simulator.py:474-583). */
+int tvbk_plan_footprint(const tvbk_plan_desc *desc, int64_t *bytes);
+
+/* Non-blocking: done = 1 when the plan's last run has finished. */
+int tvbk_poll(tvbk_plan *plan, int32_t *done);
+
 /* Replaces BaseHistory.from_simulator's result (history.py:74-158,
  * :192-195): the history of instance `inst` as history[H][n_channel][n_node]
  * (channel values, i.e. the cvars, or the coupling's pre-synaptic transform

@@ -255,13 +255,14 @@ static int layout_lds(tvbk_plan* p, int T, int64_t max_lds, bool allow_csr) {
   p->lds_c = (int)off;
   off += 2 * (int64_t)N * p->nct * T * p->esize;
   off = align_up(off, 16);
-  const int64_t csr = align_up(p->max_nnz * 4, 16) + align_up(p->max_nnz * p->esize, 16) + align_up((N + 1) * 4, 16);
+  const int64_t ne = p->max_nnz + TVBK_ROW_PAD;
+  const int64_t csr = align_up(ne * 4, 16) + align_up(ne * p->esize, 16) + align_up((N + 1) * 4, 16);
   p->csr_in_lds = allow_csr && (off + csr <= max_lds);
   if (p->csr_in_lds) {
     p->lds_edge = (int)off;
-    off += align_up(p->max_nnz * 4, 16);
+    off += align_up(ne * 4, 16);
     p->lds_w = (int)off;
-    off += align_up(p->max_nnz * p->esize, 16);
+    off += align_up(ne * p->esize, 16);
     p->lds_rp = (int)off;
     off += align_up((N + 1) * 4, 16);
   }
@@ -269,9 +270,9 @@ static int layout_lds(tvbk_plan* p, int T, int64_t max_lds, bool allow_csr) {
   return off <= max_lds ? TVBK_OK : TVBK_EUNSUPPORTED;
 }
 
-int tvbk_plan_create(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out) {
-  if (!desc || !out) return fail(nullptr, TVBK_EINVAL, "null argument");
-  *out = nullptr;
+static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out, int64_t* footprint) {
+  if (!desc || (!out && !footprint)) return fail(nullptr, TVBK_EINVAL, "null argument");
+  if (out) *out = nullptr;
   if (desc->abi_version != TVBK_ABI_VERSION) return fail(nullptr, TVBK_EINVAL, "ABI version mismatch");
   tvbk_plan* p = new tvbk_plan();
   p->d = *desc;
@@ -327,9 +328,11 @@ int tvbk_plan_create(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out) 
     cd.e_off = (int64_t)edges.size();
     cd.d_min = cn.horizon;
     if (cn.row_ptr[0] != 0 || cn.row_ptr[cn.n_node] != cn.nnz) { p->err = "row_ptr inconsistent with nnz"; return bail(TVBK_EINVAL); }
+    cd.max_row = 0;
     for (int i = 0; i < cn.n_node; ++i) {
       const int n = cn.row_ptr[i + 1] - cn.row_ptr[i];
       if (n < 0 || n > 1025) { p->err = "CSR rows must have 0..1025 entries"; return bail(TVBK_EINVAL); }
+      cd.max_row = std::max(cd.max_row, n);
     }
     for (int i = 0; i <= cn.n_node; ++i) rp.push_back(cn.row_ptr[i]);
     for (int64_t e = 0; e < cn.nnz; ++e) {
@@ -345,6 +348,11 @@ int tvbk_plan_create(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out) 
     p->max_n = std::max(p->max_n, cn.n_node);
     p->max_nnz = std::max<int64_t>(p->max_nnz, cn.nnz);
     p->conns.push_back(cd);
+  }
+  // padding past the last edge: rows are read in fixed-width blocks
+  for (int k = 0; k < TVBK_ROW_PAD; ++k) {
+    edges.push_back(0);
+    w64.push_back(0.0);
   }
   std::vector<int> inst_conn(d.n_inst, 0);
   bool shared = true;
@@ -374,6 +382,7 @@ int tvbk_plan_create(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out) 
   HIPCHK(p, hipGetDevice(&dev));
   int max_lds = 65536;
   hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  max_lds -= 2048;   // the kernels' static LDS (tile and monitor configuration)
   int kernel = d.kernel;
   if (kernel == TVBK_KERNEL_AUTO) kernel = (shared && d.n_inst >= 32) ? TVBK_KERNEL_BATCH_LANE : TVBK_KERNEL_EDGE_LANE;
   if (kernel == TVBK_KERNEL_DENSE_MFMA) { p->err = "dense MFMA kernel not available in this build"; return bail(TVBK_EUNSUPPORTED); }
@@ -444,7 +453,10 @@ int tvbk_plan_create(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out) 
     }
     td.ring_off = ring_off;
     ring_off += (int64_t)cd.horizon * cd.n_node * p->nch * T;
-    if ((int64_t)cd.horizon * cd.n_node * p->nch * T >= (1LL << 31)) { p->err = "tile history exceeds 2^31 elements"; return bail(TVBK_EUNSUPPORTED); }
+    if ((int64_t)cd.horizon * cd.n_node * p->nch * T * p->esize >= (1LL << 31)) {
+      p->err = "tile history exceeds 2 GiB";
+      return bail(TVBK_EUNSUPPORTED);
+    }
     td.img_off = img_off;
     img_off += align_up(p->img_bytes, 256);
     for (int m = 0; m < d.n_mon; ++m)
@@ -474,6 +486,29 @@ int tvbk_plan_create(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out) 
         o[p->mi.ndp + p->ncp + v] = p->stochastic ? std::sqrt(2.0 * raw[p->mi.nmp + p->ncp + v]) : 0.0;
     }
 
+  // ---- device footprint (dry run stops here)
+  {
+    size_t tot = p->conns.size() * sizeof(ConnDesc) + edges.size() * 4 + 16 + rp.size() * 4 +
+                 (w64.size() * p->esize + 16) + prm.size() * p->esize + (d.inst_stream ? (size_t)B * 8 : 0) +
+                 (size_t)ring_off * p->esize + (size_t)img_off + (size_t)bacc_off * 8 + (size_t)B * 4 +
+                 p->tiles.size() * sizeof(TileDesc);
+    for (int m = 0; m < d.n_mon; ++m) {
+      if (d.mon[m].kind == TVBK_MON_BOLD) tot += p->taps_host[m].size() * 8;
+      const int64_t cap = d.max_steps / d.mon[m].istep + 1;
+      int64_t elems = 0;
+      if (T > 1) elems = cap * d.mon[m].n_voi * (int64_t)p->conns[0].n_node * B;
+      else
+        for (int b = 0; b < B; ++b) elems += cap * d.mon[m].n_voi * (int64_t)p->conns[inst_conn[b]].n_node;
+      tot += (size_t)elems * (d.mon[m].kind == TVBK_MON_BOLD ? 8 : p->esize);
+    }
+    if (d.noise == TVBK_NOISE_INJECTED) tot += (size_t)d.max_steps * p->mi.nsv * p->max_n * B * 8;
+    if (footprint) *footprint = (int64_t)tot;
+    if (!out) {
+      destroy(p);
+      return TVBK_OK;
+    }
+  }
+
   // ---- allocate and upload
   int rc;
   auto up = [&](DevBuf* b, const void* src, size_t bytes) -> int {
@@ -486,14 +521,14 @@ int tvbk_plan_create(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out) 
     return TVBK_OK;
   };
   if ((rc = up(&p->d_conns, p->conns.data(), p->conns.size() * sizeof(ConnDesc)))) return bail(rc);
-  if ((rc = up(&p->d_edges, edges.data(), edges.size() * 4 + 16))) return bail(rc);
+  if ((rc = up(&p->d_edges, edges.data(), edges.size() * 4))) return bail(rc);
   if ((rc = up(&p->d_rp, rp.data(), rp.size() * 4))) return bail(rc);
   if (p->esize == 8) {
-    if ((rc = up(&p->d_w, w64.data(), w64.size() * 8 + 16))) return bail(rc);
+    if ((rc = up(&p->d_w, w64.data(), w64.size() * 8))) return bail(rc);
     if ((rc = up(&p->d_params, prm.data(), prm.size() * 8))) return bail(rc);
   } else {
     std::vector<float> w32(w64.begin(), w64.end()), p32(prm.begin(), prm.end());
-    if ((rc = up(&p->d_w, w32.data(), w32.size() * 4 + 16))) return bail(rc);
+    if ((rc = up(&p->d_w, w32.data(), w32.size() * 4))) return bail(rc);
     if ((rc = up(&p->d_params, p32.data(), p32.size() * 4))) return bail(rc);
   }
   if (d.inst_stream) {
@@ -528,6 +563,25 @@ int tvbk_plan_create(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out) 
   HIPCHK(p, hipStreamSynchronize(p->stream));
   *out = p;
   return TVBK_OK;
+}
+
+int tvbk_plan_create(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out) {
+  if (!out) return fail(nullptr, TVBK_EINVAL, "null argument");
+  return plan_build(desc, stream, out, nullptr);
+}
+
+int tvbk_plan_footprint(const tvbk_plan_desc* desc, int64_t* bytes) {
+  if (!bytes) return fail(nullptr, TVBK_EINVAL, "null argument");
+  return plan_build(desc, nullptr, nullptr, bytes);
+}
+
+int tvbk_poll(tvbk_plan* p, int32_t* done) {
+  if (!p || !done) return fail(p, TVBK_EINVAL, "null argument");
+  if (!p->have_timing) { *done = 1; return TVBK_OK; }
+  hipError_t e = hipEventQuery(p->ev1);
+  if (e == hipSuccess) { *done = 1; return TVBK_OK; }
+  if (e == hipErrorNotReady) { *done = 0; return TVBK_OK; }
+  return fail(p, TVBK_EHIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
 }
 
 int tvbk_plan_shape(tvbk_plan* p, int32_t* kernel, int32_t* tile, int32_t* waves, int32_t* lds_bytes) {

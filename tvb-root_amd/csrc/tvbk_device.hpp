@@ -159,14 +159,11 @@ __device__ inline R pw_sum(int s, int n, F& f) {
   }
 }
 
-// Same order for rows of at most RMAX entries, with every load of the row
-// issued before the first add (the row is staged in registers): one memory
-// round trip per row instead of one per block of eight.
-template <int RMAX, class R, class F>
-__device__ inline R reduceat_row_regs(int n, F& f) {
-  R v[RMAX];
-#pragma unroll
-  for (int e = 0; e < RMAX; ++e) v[e] = (e < n) ? f(e) : R(0);
+// Same order for rows of at most RMAX entries whose values are already
+// staged in registers v[0..n-1]: every load of the row is issued before the
+// first add, one memory round trip per row.
+template <int RMAX, class R>
+__device__ inline R reduceat_regs(int n, const R* v) {
   if (n <= 1) return n == 1 ? v[0] : R(0);
   const int m = n - 1;                       // pairwise part v[1..n-1]
   R res;
@@ -192,6 +189,82 @@ __device__ inline R reduceat_row_regs(int n, F& f) {
       if (i >= nfull && i < m) res += v[1 + i];
   }
   return v[0] + res;
+}
+
+// numpy add.reduceat order for rows longer than the register path: the
+// pairwise blocks of eight are loaded as a batch before they are used.
+// ld(e) returns the raw value of entry e (zero past the row end), wt(e) its
+// weight; products are rounded before they are summed, as numpy does.
+template <class R, class LD, class WT>
+__device__ inline R pw_leaf_blocked(int s, int n, LD& ld, WT& wt) {
+  R x[8];
+  if (n < 8) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = ld(s + k);
+    R res = R(0);
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+      if (k < n) res += wt(s + k) * x[k];
+    return res;
+  }
+  R r[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[k] = ld(s + k);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r[k] = wt(s + k) * x[k];
+  const int nfull = n - (n & 7);
+  int i = 8;
+  for (; i < nfull; i += 8) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = ld(s + i + k);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) r[k] += wt(s + i + k) * x[k];
+  }
+  R res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  const int rem = n - nfull;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[k] = ld(s + i + k);
+#pragma unroll
+  for (int k = 0; k < 7; ++k)
+    if (k < rem) res += wt(s + i + k) * x[k];
+  return res;
+}
+
+template <int DEPTH, class R, class LD, class WT>
+__device__ inline R pw_sum_blocked(int s, int n, LD& ld, WT& wt) {
+  if constexpr (DEPTH == 0) {
+    return pw_leaf_blocked<R>(s, n, ld, wt);
+  } else {
+    if (n <= 128) return pw_leaf_blocked<R>(s, n, ld, wt);
+    int n2 = n / 2;
+    n2 -= n2 & 7;
+    R a = pw_sum_blocked<DEPTH - 1, R>(s, n2, ld, wt);
+    R b = pw_sum_blocked<DEPTH - 1, R>(s + n2, n - n2, ld, wt);
+    return a + b;
+  }
+}
+
+template <class R, class LD, class WT>
+__device__ inline R reduceat_row_blocked(int n, LD& ld, WT& wt) {
+  if (n <= 0) return R(0);
+  const R first = wt(0) * ld(0);
+  if (n == 1) return first;
+  auto ld1 = [&](int e) -> R { return ld(e + 1); };
+  auto wt1 = [&](int e) -> R { return wt(e + 1); };
+  return first + pw_sum_blocked<3, R>(0, n - 1, ld1, wt1);
+}
+
+// Raw buffer loads: 32-bit byte offsets from a per-tile base; offsets past
+// num_records read as zero without a memory access.
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+template <class R> __device__ inline R buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <> __device__ inline float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <> __device__ inline double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
 }
 
 template <class R, class F>

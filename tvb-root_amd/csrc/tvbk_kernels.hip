@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ KCfg kc;
 
-  if (a.tiles[blockIdx.x].n_steps <= 0) return;
+  if (a.tiles[blockIdx.x].n_steps <= 0) return;   // uniform: whole block leaves
   if (threadIdx.x == 0) {
     kc.td = a.tiles[blockIdx.x];
     for (int m = 0; m < TVBK_KMAX_MON; ++m) kc.mon[m] = a.mon[m];
@@ -68,10 +68,20 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs a) {
     }
   }
   __syncthreads();
+  // scalars of this tile and its connectome: loaded straight from global
+  // memory (uniform addresses -> scalar loads, values stay in SGPRs); the
+  // per-monitor addressing is read from the LDS copy kc.td when used.
+  const TileDesc* tdg = a.tiles + blockIdx.x;
   const TileDesc& td = kc.td;
-  const ConnDesc cd = a.conns[td.conn];
+  const int conn = tdg->conn;
+  const ConnDesc* cdg = a.conns + conn;
+  const int N = cdg->n_node, H = cdg->horizon;
+  const int cd_nnz = cdg->nnz, cd_max_row = cdg->max_row;
+  const int64_t cd_e_off = cdg->e_off, cd_rp_off = cdg->rp_off;
+  const int td_inst0 = tdg->inst0, td_n_inst = tdg->n_inst;
+  const int64_t td_step0 = tdg->step0, td_n_steps = tdg->n_steps, td_ring_off = tdg->ring_off,
+                td_img_off = tdg->img_off;
   const int T = a.T;
-  const int N = cd.n_node, H = cd.horizon;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6, nwave = nthr >> 6;
 
@@ -80,29 +90,25 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs a) {
   double* s_bi = reinterpret_cast<double*>(smem + a.lds_bold);
   R* s_x = reinterpret_cast<R*>(smem + a.lds_state);
   R* s_c = reinterpret_cast<R*>(smem + a.lds_c);
-  const uint32_t* g_edge = a.edges + cd.e_off;
-  const R* g_w = reinterpret_cast<const R*>(a.weights) + cd.e_off;
-  const int32_t* g_rp = a.row_ptr + cd.rp_off;
-  const uint32_t* e_edge = g_edge;
-  const R* e_w = g_w;
-  const int32_t* e_rp = g_rp;
-  const int nnz = cd.nnz;
+  const uint32_t* g_edge = a.edges + cd_e_off;
+  const R* g_w = reinterpret_cast<const R*>(a.weights) + cd_e_off;
+  const int32_t* g_rp = a.row_ptr + cd_rp_off;
+  uint32_t* l_edge = reinterpret_cast<uint32_t*>(smem + a.lds_edge);
+  R* l_w = reinterpret_cast<R*>(smem + a.lds_w);
+  int32_t* l_rp = reinterpret_cast<int32_t*>(smem + a.lds_rp);
+  const int nnz = cd_nnz;
   if (a.csr_in_lds) {
-    uint32_t* l_edge = reinterpret_cast<uint32_t*>(smem + a.lds_edge);
-    R* l_w = reinterpret_cast<R*>(smem + a.lds_w);
-    int32_t* l_rp = reinterpret_cast<int32_t*>(smem + a.lds_rp);
-    for (int k = tid; k < nnz; k += nthr) { l_edge[k] = g_edge[k]; l_w[k] = g_w[k]; }
+    for (int k = tid; k < nnz + TVBK_ROW_PAD; k += nthr) { l_edge[k] = g_edge[k]; l_w[k] = g_w[k]; }
     for (int k = tid; k <= N; k += nthr) l_rp[k] = g_rp[k];
-    e_edge = l_edge; e_w = l_w; e_rp = l_rp;
   }
-  copy_bytes<R>(img, a.img + td.img_off, a.img_bytes);
+  copy_bytes<R>(img, a.img + td_img_off, a.img_bytes);
   __syncthreads();
 
   // ---- per-lane instance parameters (phase B lane and phase A lane agree:
   //      blockDim and 64 are multiples of T) -------------------------------
   const int t = tid % T;
-  const bool live = t < td.n_inst;
-  const int inst = td.inst0 + (live ? t : 0);
+  const bool live = t < td_n_inst;
+  const int inst = td_inst0 + (live ? t : 0);
   const int64_t spatial_row = (int64_t)N * a.prow;   // per-node params: one prow row per node
   const R* prow = reinterpret_cast<const R*>(a.params) + (int64_t)inst * (a.spatial ? spatial_row : a.prow);
   R cp[NCP > 0 ? NCP : 1];
@@ -117,11 +123,11 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs a) {
   const R dt = (R)a.dt;
   const uint64_t stream = a.streams ? a.streams[inst] : (uint64_t)inst;
 
-  R* ring = reinterpret_cast<R*>(a.ring) + td.ring_off;
+  R* ring = reinterpret_cast<R*>(a.ring) + td_ring_off;
   const int rstride = NCH * T;   // elements per (slot, node)
 
   // ---- step counters (wave-uniform) --------------------------------------
-  int64_t s = td.step0;
+  int64_t s = td_step0;
   int smod = (int)((s - 1) % H);          // (s-1) % H ; s >= 1
   if (threadIdx.x == 0) {
     for (int m = 0; m < a.n_mon; ++m) {
@@ -137,27 +143,48 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs a) {
   const int P = N * NCH;
   const int NG = 64 / T;        // lane groups per wave
   const int g = lane / T;
-  const int nsteps = (int)td.n_steps;
+  const int nsteps = (int)td_n_steps;
 
-  for (int k = 0; k < nsteps; ++k, ++s) {
-    const int buf = (int)(k & 1);
-    R* cbuf = s_c + (int64_t)buf * N * NCT * T;
-    // ================= phase A: coupling of step s ========================
+  // ---- phase A: per (node, channel) pair, a group of T lanes reduces the
+  // node's CSR row; the history is gathered with buffer loads at byte offset
+  // slot*sstride + col*cstride + lane_off.  Instantiated once for LDS and
+  // once for global CSR arrays so every access has a known address space.
+  constexpr int RMAX = sizeof(R) == 4 ? 32 : 24;
+  const uint32_t cstride = (uint32_t)rstride * (uint32_t)sizeof(R);
+  const uint32_t sstride = (uint32_t)N * cstride;
+  const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(ring, (uint32_t)H * sstride);
+  const int rmax = cd_max_row < RMAX ? cd_max_row : RMAX;
+  auto phase_a = [&](const auto* E, const auto* W, const auto* RP, R* cbuf) {
     for (int q = wave * NG + g; q - g < P; q += nwave * NG) {
       if (q < P) {
         const int i = q / NCH, ch = q - (q / NCH) * NCH;
-        const int e0 = e_rp[i];
-        const int n = e_rp[i + 1] - e0;
-        const R* rb = ring + ch * T + t;
-        auto prod = [&](int e) -> R {
-          const uint32_t ed = e_edge[e0 + e];
-          const int j = (int)(ed & 0xffffu), d = (int)(ed >> 16);
-          int sl = smod - d;
+        const int e0 = RP[i];
+        const int n = RP[i + 1] - e0;
+        const uint32_t lane_off = (uint32_t)(ch * T + t) * (uint32_t)sizeof(R);
+        auto offset = [&](uint32_t ed) -> uint32_t {
+          int sl = smod - (int)(ed >> 16);
           sl += (sl < 0) ? H : 0;
-          return e_w[e0 + e] * rb[(sl * N + j) * rstride];
+          return (uint32_t)sl * sstride + (ed & 0xffffu) * cstride + lane_off;
         };
-        constexpr int RMAX = sizeof(R) == 4 ? 32 : 24;
-        R acc = (n <= RMAX) ? reduceat_row_regs<RMAX, R>(n, prod) : reduceat_row<R>(n, prod);
+        R acc;
+        if (n <= RMAX) {
+          // three passes: offsets (LDS + ALU), all loads back to back, then
+          // the weights -- no load result is used before the last load issues
+          uint32_t off[RMAX];
+#pragma unroll
+          for (int e = 0; e < RMAX; ++e) off[e] = (e < n) ? offset(E[e0 + e]) : 0xFFFFFFF0u;
+          R v[RMAX];
+#pragma unroll
+          for (int e = 0; e < RMAX; ++e) v[e] = buf_load<R>(rsrc, off[e]);
+#pragma unroll
+          for (int e = 0; e < RMAX; ++e) v[e] = W[e0 + e] * v[e];
+          acc = reduceat_regs<RMAX, R>(n, v);
+        } else {
+          // long rows: the same order in blocks of eight loads
+          auto ld = [&](int e) -> R { return buf_load<R>(rsrc, e < n ? offset(E[e0 + e]) : 0xFFFFFFF0u); };
+          auto wt = [&](int e) -> R { return W[e0 + e]; };
+          acc = reduceat_row_blocked<R>(n, ld, wt);
+        }
         if (CF == CF_SIGMOID_JR) {
           cbuf[(i * NCT) * T + t] = C::post(acc, cp);
         } else {
@@ -165,6 +192,14 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs a) {
         }
       }
     }
+  };
+
+  for (int k = 0; k < nsteps; ++k, ++s) {
+    const int buf = (int)(k & 1);
+    R* cbuf = s_c + (int64_t)buf * N * NCT * T;
+    // ================= phase A: coupling of step s ========================
+    if (a.csr_in_lds) phase_a(l_edge, l_w, l_rp, cbuf);
+    else phase_a(g_edge, g_w, g_rp, cbuf);
     __syncthreads();
     // ================= phase B: integrate, write history, monitors =========
     const int wslot = (smod + 1 == H) ? 0 : smod + 1;   // s % H
@@ -311,7 +346,7 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs a) {
   // note: msample counts emissions of this call; the host derives the same
   // count from the step range.
   __syncthreads();
-  copy_bytes<R>(a.img + td.img_off, img, a.img_bytes);
+  copy_bytes<R>(a.img + td_img_off, img, a.img_bytes);
   if (bad && live) atomicOr(a.flags + inst, 1);
 }
 

@@ -10,9 +10,13 @@ namespace tvbk {
 
 // One connectome as uploaded: CSR rows of packed (col | delay << 16) edges.
 struct ConnDesc {
-  int32_t n_node, horizon, nnz, d_min;
+  int32_t n_node, horizon, nnz, d_min, max_row, pad;
   int64_t rp_off, e_off;   // element offsets into the concatenated arrays
 };
+
+// CSR arrays (global and LDS copies) carry this many padding entries past
+// the last edge so that rows can be read in fixed-width blocks.
+#define TVBK_ROW_PAD 32
 
 // One workgroup's tile of instances and where its data lives.
 struct TileDesc {

@@ -72,6 +72,8 @@ def _bind(lib):
         "tvbk_model_info": ([C.c_int32, C.c_int32, P, P, P, P], C.c_int),
         "tvbk_plan_create": ([C.POINTER(PlanDesc), P, C.POINTER(C.c_void_p)], C.c_int),
         "tvbk_plan_destroy": ([P], C.c_int),
+        "tvbk_plan_footprint": ([C.POINTER(PlanDesc), P], C.c_int),
+        "tvbk_poll": ([P, P], C.c_int),
         "tvbk_set_history": ([P, C.c_int32, P, C.c_int32, P, C.c_int64], C.c_int),
         "tvbk_init_uniform": ([P, C.c_uint64, P, P], C.c_int),
         "tvbk_set_noise": ([P, P, C.c_int64], C.c_int),
@@ -97,6 +99,7 @@ def _bind(lib):
 
 EXPORTED = (
     "tvbk_abi_version", "tvbk_model_info", "tvbk_plan_create", "tvbk_plan_destroy",
+    "tvbk_plan_footprint", "tvbk_poll",
     "tvbk_set_history", "tvbk_init_uniform", "tvbk_set_noise", "tvbk_run_steps",
     "tvbk_set_steps", "tvbk_sync", "tvbk_monitor_layout", "tvbk_read_monitor",
     "tvbk_monitor_device_ptr", "tvbk_export_state", "tvbk_read_flags",

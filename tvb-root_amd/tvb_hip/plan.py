@@ -64,7 +64,7 @@ class Plan:
     def __init__(self, *, model, cfun, scheme, precision, noise, conns, params, dt,
                  obs_coef, monitors, max_steps, inst_conn=None, node_params=None,
                  node_param_mask=0, seed=0, inst_stream=None, kernel=L.KERNEL_AUTO,
-                 tile=0, waves=0, stream=None):
+                 tile=0, waves=0, stream=None, _dry=False):
         self._lib = L.load()
         self.handle = None
         self.model, self.cfun, self.scheme = model, cfun, scheme
@@ -113,14 +113,30 @@ class Plan:
         d.max_steps = self.max_steps
         d.tile, d.waves = tile, waves
         self._desc = d
+        self.esize = 8 if precision == L.F64 else 4
+        self._z_keep = None
+        if _dry:
+            nb = C.c_int64()
+            L.check(self._lib.tvbk_plan_footprint(C.byref(d), C.byref(nb)))
+            self.footprint_bytes = nb.value
+            return
         h = C.c_void_p()
         L.check(self._lib.tvbk_plan_create(C.byref(d), stream, C.byref(h)))
         self.handle = h
         k, t, w, lds = (C.c_int32() for _ in range(4))
         L.check(self._lib.tvbk_plan_shape(h, C.byref(k), C.byref(t), C.byref(w), C.byref(lds)), h)
         self.kernel, self.tile, self.waves, self.lds_bytes = k.value, t.value, w.value, lds.value
-        self.esize = 8 if precision == L.F64 else 4
-        self._z_keep = None
+
+    @classmethod
+    def footprint(cls, **kw):
+        """Device bytes the plan would allocate (no allocation)."""
+        return cls(_dry=True, **kw).footprint_bytes
+
+    def done(self):
+        """Non-blocking: True once the last run has finished on the device."""
+        flag = C.c_int32()
+        self._chk(self._lib.tvbk_poll(self.handle, C.byref(flag)))
+        return bool(flag.value)
 
     # ------------------------------------------------------------------
     def _chk(self, rc):

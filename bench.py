@@ -214,14 +214,16 @@ def run_c4(n_requests=24, rate_hz=40.0, duration_scale=0.01, inst_range=(1, 64),
     N log-uniform in [68, 998], instances in inst_range, durations
     U[5, 60] s x duration_scale), admitted by device footprint."""
     import torch
+    from tvb_hip import _lib
     from tvb_hip.serve import Server, synthetic_requests
     reqs = synthetic_requests(n_requests, rate_hz, seed=seed, inst_range=inst_range,
                               duration_scale=duration_scale)
-    streams = [torch.cuda.Stream() for _ in range(4)]
+    streams = [_lib.stream_create() for _ in range(4)]
     free, total = torch.cuda.mem_get_info()
-    srv = Server(hbm_budget=int(0.5 * free), streams=[s.cuda_stream for s in streams], chunk_steps=2000,
-                 keep_outputs=True)
+    srv = Server(hbm_budget=int(0.5 * free), streams=streams, chunk_steps=2000, keep_outputs=True)
     stats = srv.run(reqs)
+    for st in streams:
+        _lib.stream_destroy(st)
     stats.update(duration_scale=duration_scale, rate_hz=rate_hz, inst_range=list(inst_range))
     return stats
 

@@ -171,6 +171,12 @@ int tvbk_plan_footprint(const tvbk_plan_desc *desc, int64_t *bytes);
 /* Non-blocking: done = 1 when the plan's last run has finished. */
 int tvbk_poll(tvbk_plan *plan, int32_t *done);
 
+/* A HIP stream (default priority) for plans of the request loop, so that
+ * callers without another GPU runtime can overlap plans; no counterpart in
+ * the reference, whose loop is single-threaded Python. */
+int tvbk_stream_create(void **stream);
+int tvbk_stream_destroy(void *stream);
+
 /* Replaces BaseHistory.from_simulator's result (history.py:74-158,
  * :192-195): the history of instance `inst` as history[H][n_channel][n_node]
  * (channel values, i.e. the cvars, or the coupling's pre-synaptic transform

@@ -241,3 +241,33 @@ def test_legacy_noise_drop_in():
     np.testing.assert_allclose(td, t)
     assert rel_err(xd, x) <= 1e-10
     assert s.current_step == 400
+
+
+def test_dense_mfma_zero_delay_sweep():
+    """All delays zero: the sweep runs on the matrix-core kernel (coupling as
+    W X per channel).  FP32 vs the fp64 oracle: TemporalAverage max rel
+    <= 1e-4; vs the sparse batch-lane kernel on the same instances (only the
+    summation order differs): max rel <= 1e-5."""
+    _say("dense mfma")
+    import tvb_hip._lib as L
+    B, nstep = 40, 2000
+    w, Lm = synthetic(20, 0.3, 5.0, 40.0, 3)
+    rng = np.random.default_rng(8)
+    cases = []
+    for b in range(B):
+        orc, sim, _ = make_case("MontbrioPazoRoxin", weights=w, tract_lengths=Lm, speed=1e9, dt=0.05,
+                                scheme="HeunStochastic", cparams=dict(a=float(rng.uniform(0.1, 0.6))),
+                                nsig=[0.01, 0.01], seed=500 + b,
+                                monitors=(("TemporalAverage", 1.0, None), ("Bold", 10.0, None)))
+        cases.append((orc, sim))
+    sims = [s for _, s in cases]
+    z = list(np.random.default_rng(6).standard_normal((B, nstep, 2, 20)))
+    be = _backend()
+    dense = be.run_batch(sims, nstep, precision="fp32", noise=z)
+    assert be.last_plan_shape[0] == L.KERNEL_DENSE_MFMA
+    sparse = be.run_batch(sims, nstep, precision="fp32", noise=z, kernel="batch")
+    for b in (0, 33, 39):
+        (_, ta), (_, bo) = cases[b][0]["fp64"].run(nstep, noise=z[b])
+        assert rel_err(dense[b][0][1], ta) <= 1e-4
+        assert rel_err(dense[b][1][1], bo) <= 1e-6
+        assert rel_err(dense[b][0][1], sparse[b][0][1]) <= 1e-5

@@ -10,9 +10,9 @@ pytestmark = pytest.mark.gpu
 
 
 def _streams(k):
-    import torch
-    ss = [torch.cuda.Stream() for _ in range(k)]
-    return ss, [s.cuda_stream for s in ss]
+    from tvb_hip import _lib
+    ss = [_lib.stream_create() for _ in range(k)]
+    return ss, ss
 
 
 def test_serving_mix_equals_alone():

@@ -134,6 +134,8 @@ struct tvbk_plan {
   // device buffers
   DevBuf d_tiles, d_conns, d_edges, d_w, d_rp, d_ring, d_img, d_bacc, d_params, d_flags, d_streams, d_z;
   DevBuf d_taps[TVBK_MAX_MON], d_out[TVBK_MAX_MON];
+  DevBuf d_dense;             // dense zero-delay kernel: W [16*RB][KP] float
+  int dense_kb = 0, dense_rb = 0, lds_x = 0;
   int64_t z_steps = 0;
   const double* z_ptr = nullptr;
   // last run bookkeeping
@@ -213,6 +215,7 @@ int tvbk_model_info(int32_t model, int32_t cfun, int32_t* n_model_param, int32_t
 const char* tvbk_last_error(tvbk_plan* plan) { return plan ? plan->err.c_str() : g_err.c_str(); }
 
 static void destroy(tvbk_plan* p) {
+  (void)hipStreamSynchronize(p->stream);   // no pending copy or kernel touches the buffers
   DevBuf* bufs[] = {&p->d_tiles, &p->d_conns, &p->d_edges, &p->d_w,      &p->d_rp,     &p->d_ring,
                     &p->d_img,   &p->d_bacc,  &p->d_params, &p->d_flags, &p->d_streams, &p->d_z};
   for (DevBuf* b : bufs) dev_free(b);
@@ -220,6 +223,7 @@ static void destroy(tvbk_plan* p) {
     dev_free(&p->d_taps[m]);
     dev_free(&p->d_out[m]);
   }
+  dev_free(&p->d_dense);
   if (p->ev0) hipEventDestroy(p->ev0);
   if (p->ev1) hipEventDestroy(p->ev1);
   delete p;
@@ -255,14 +259,14 @@ static int layout_lds(tvbk_plan* p, int T, int64_t max_lds, bool allow_csr) {
   p->lds_c = (int)off;
   off += 2 * (int64_t)N * p->nct * T * p->esize;
   off = align_up(off, 16);
+  // CSR staged as 16-byte records {col offset, delay, weight bits}
   const int64_t ne = p->max_nnz + TVBK_ROW_PAD;
-  const int64_t csr = align_up(ne * 4, 16) + align_up(ne * p->esize, 16) + align_up((N + 1) * 4, 16);
+  const int64_t csr = ne * 16 + align_up((N + 1) * 4, 16);
   p->csr_in_lds = allow_csr && (off + csr <= max_lds);
   if (p->csr_in_lds) {
     p->lds_edge = (int)off;
-    off += align_up(ne * 4, 16);
-    p->lds_w = (int)off;
-    off += align_up(ne * p->esize, 16);
+    off += ne * 16;
+    p->lds_w = 0;
     p->lds_rp = (int)off;
     off += align_up((N + 1) * 4, 16);
   }
@@ -384,12 +388,34 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
   max_lds -= 2048;   // the kernels' static LDS (tile and monitor configuration)
   int kernel = d.kernel;
-  if (kernel == TVBK_KERNEL_AUTO) kernel = (shared && d.n_inst >= 32) ? TVBK_KERNEL_BATCH_LANE : TVBK_KERNEL_EDGE_LANE;
-  if (kernel == TVBK_KERNEL_DENSE_MFMA) { p->err = "dense MFMA kernel not available in this build"; return bail(TVBK_EUNSUPPORTED); }
+  // dense zero-delay kernel: every delay zero (horizon 1), one shared
+  // connectome of at most 128 nodes, Linear coupling, float32, G2D / MPR
+  const bool dense_ok = shared && d.precision == TVBK_F32 && d.cfun == TVBK_CFUN_LINEAR &&
+                        (d.model == TVBK_MODEL_G2D || d.model == TVBK_MODEL_MPR) && p->conns.size() == 1 &&
+                        p->conns[0].horizon == 1 && p->conns[0].n_node <= 128;
+  if (kernel == TVBK_KERNEL_AUTO) {
+    if (dense_ok && d.n_inst >= 32) kernel = TVBK_KERNEL_DENSE_MFMA;
+    else kernel = (shared && d.n_inst >= 32) ? TVBK_KERNEL_BATCH_LANE : TVBK_KERNEL_EDGE_LANE;
+  }
+  if (kernel == TVBK_KERNEL_DENSE_MFMA && !dense_ok) {
+    p->err = "dense MFMA kernel needs float32, Linear coupling, G2D or MPR, one connectome of <= 128 nodes "
+             "with all delays zero";
+    return bail(TVBK_EUNSUPPORTED);
+  }
   if (kernel == TVBK_KERNEL_BATCH_LANE && !shared) { p->err = "batch-lane kernel needs one shared connectome"; return bail(TVBK_EINVAL); }
   p->kernel = kernel;
   int T = 1;
-  if (kernel == TVBK_KERNEL_BATCH_LANE) {
+  if (kernel == TVBK_KERNEL_DENSE_MFMA) {
+    if (d.tile && d.tile != 32) { p->err = "dense kernel uses tile 32"; return bail(TVBK_EINVAL); }
+    T = 32;
+    layout_lds(p, T, max_lds, false);
+    const int N0 = p->conns[0].n_node;
+    p->dense_rb = (N0 + 15) / 16;
+    p->dense_kb = (N0 + 3) / 4;
+    p->lds_x = p->lds_c;
+    p->lds_total = p->lds_x + 2 * p->nch * (4 * p->dense_kb) * T * 4;
+    if (p->lds_total > max_lds) { p->err = "dense tile does not fit in LDS"; return bail(TVBK_EUNSUPPORTED); }
+  } else if (kernel == TVBK_KERNEL_BATCH_LANE) {
     if (d.tile) {
       T = d.tile;
     } else {
@@ -411,10 +437,16 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   {
     const int P = p->max_n * std::max(p->nch, 1);
     int nw;
-    if (d.waves) nw = d.waves;
+    if (kernel == TVBK_KERNEL_DENSE_MFMA) nw = 2 * p->dense_rb;
+    else if (d.waves) nw = d.waves;
     else if (T > 1) nw = std::min(16, std::max(1, (P + 64 / T - 1) / (64 / T)));
     else nw = std::min(16, std::max(1, (std::max(P, p->max_n) + 63) / 64));
-    if (nw < 1 || nw > 16) { p->err = "waves must be in [1, 16]"; return bail(TVBK_EINVAL); }
+    const int max_waves = p->esize == 8 ? 8 : 16;   // the float64 kernels are built for 512 threads
+    if (!d.waves) nw = std::min(nw, max_waves);
+    if (nw < 1 || nw > max_waves) {
+      p->err = "waves must be in [1, " + std::to_string(max_waves) + "]";
+      return bail(TVBK_EINVAL);
+    }
     p->waves = nw;
     p->nthreads = 64 * nw;
   }
@@ -455,6 +487,10 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
     ring_off += (int64_t)cd.horizon * cd.n_node * p->nch * T;
     if ((int64_t)cd.horizon * cd.n_node * p->nch * T * p->esize >= (1LL << 31)) {
       p->err = "tile history exceeds 2 GiB";
+      return bail(TVBK_EUNSUPPORTED);
+    }
+    if ((int64_t)cd.n_node * p->nch * T * p->esize >= (1LL << 24)) {
+      p->err = "one history slot of a tile exceeds 16 MiB";
       return bail(TVBK_EUNSUPPORTED);
     }
     td.img_off = img_off;
@@ -511,11 +547,14 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
 
   // ---- allocate and upload
   int rc;
+  // zeroing and uploads are ordered on the plan's stream (which may be a
+  // non-blocking stream, unordered with the null stream); the host arrays
+  // stay alive until the stream synchronisation at the end of creation
   auto up = [&](DevBuf* b, const void* src, size_t bytes) -> int {
     int r = dev_alloc(p, b, bytes);
     if (r) return r;
     if (bytes) {
-      hipError_t e = hipMemcpy(b->p, src, bytes, hipMemcpyHostToDevice);
+      hipError_t e = hipMemcpyAsync(b->p, src, bytes, hipMemcpyHostToDevice, p->stream);
       if (e != hipSuccess) return fail(p, TVBK_EHIP, std::string("upload: ") + hipGetErrorString(e));
     }
     return TVBK_OK;
@@ -523,16 +562,26 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   if ((rc = up(&p->d_conns, p->conns.data(), p->conns.size() * sizeof(ConnDesc)))) return bail(rc);
   if ((rc = up(&p->d_edges, edges.data(), edges.size() * 4))) return bail(rc);
   if ((rc = up(&p->d_rp, rp.data(), rp.size() * 4))) return bail(rc);
+  std::vector<float> w32, p32, wd;   // host sources of async uploads: alive until the final sync
   if (p->esize == 8) {
     if ((rc = up(&p->d_w, w64.data(), w64.size() * 8))) return bail(rc);
     if ((rc = up(&p->d_params, prm.data(), prm.size() * 8))) return bail(rc);
   } else {
-    std::vector<float> w32(w64.begin(), w64.end()), p32(prm.begin(), prm.end());
+    w32.assign(w64.begin(), w64.end());
+    p32.assign(prm.begin(), prm.end());
     if ((rc = up(&p->d_w, w32.data(), w32.size() * 4))) return bail(rc);
     if ((rc = up(&p->d_params, p32.data(), p32.size() * 4))) return bail(rc);
   }
   if (d.inst_stream) {
     if ((rc = up(&p->d_streams, d.inst_stream, (size_t)B * 8))) return bail(rc);
+  }
+  if (kernel == TVBK_KERNEL_DENSE_MFMA) {
+    const tvbk_connectome& cn = d.conns[0];
+    const int KP = 4 * p->dense_kb, NR = 16 * p->dense_rb;
+    wd.assign((size_t)NR * KP, 0.0f);
+    for (int i = 0; i < cn.n_node; ++i)
+      for (int e = cn.row_ptr[i]; e < cn.row_ptr[i + 1]; ++e) wd[(size_t)i * KP + cn.col[e]] = (float)cn.weight[e];
+    if ((rc = up(&p->d_dense, wd.data(), wd.size() * 4))) return bail(rc);
   }
   for (int m = 0; m < d.n_mon; ++m)
     if (d.mon[m].kind == TVBK_MON_BOLD)
@@ -573,6 +622,22 @@ int tvbk_plan_create(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out) 
 int tvbk_plan_footprint(const tvbk_plan_desc* desc, int64_t* bytes) {
   if (!bytes) return fail(nullptr, TVBK_EINVAL, "null argument");
   return plan_build(desc, nullptr, nullptr, bytes);
+}
+
+int tvbk_stream_create(void** stream) {
+  if (!stream) return fail(nullptr, TVBK_EINVAL, "null argument");
+  hipStream_t st = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e != hipSuccess) return fail(nullptr, TVBK_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  *stream = (void*)st;
+  return TVBK_OK;
+}
+
+int tvbk_stream_destroy(void* stream) {
+  if (!stream) return TVBK_OK;
+  hipError_t e = hipStreamDestroy((hipStream_t)stream);
+  if (e != hipSuccess) return fail(nullptr, TVBK_EHIP, std::string("hipStreamDestroy: ") + hipGetErrorString(e));
+  return TVBK_OK;
 }
 
 int tvbk_poll(tvbk_plan* p, int32_t* done) {
@@ -863,9 +928,16 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
     ma.taps = (const double*)p->d_taps[m].p;
     ma.lds_off = p->mon_lds_off[m];
   }
+  a.dense_w = (const float*)p->d_dense.p;
+  a.dense_kb = p->dense_kb;
+  a.dense_rb = p->dense_rb;
+  a.lds_x = p->lds_x;
   HIPCHK(p, hipEventRecord(p->ev0, p->stream));
-  HIPCHK(p, launch_step(d.precision, d.model, d.cfun, a, (int)p->tiles.size(), p->nthreads, (size_t)p->lds_total,
-                        p->stream));
+  if (p->kernel == TVBK_KERNEL_DENSE_MFMA)
+    HIPCHK(p, launch_dense(d.model, a, (int)p->tiles.size(), p->nthreads, (size_t)p->lds_total, p->stream));
+  else
+    HIPCHK(p, launch_step(d.precision, d.model, d.cfun, a, (int)p->tiles.size(), p->nthreads, (size_t)p->lds_total,
+                          p->stream));
   HIPCHK(p, hipEventRecord(p->ev1, p->stream));
   p->have_timing = true;
   for (size_t k = 0; k < p->tiles.size(); ++k) {

@@ -57,6 +57,10 @@ struct StepArgs {
   uint32_t key0, key1;
   double dt, sqrt_dt;
   int32_t lds_bold, lds_state, lds_c, lds_edge, lds_w, lds_rp;
+  // dense zero-delay kernel: W padded to [16*RB][KP] (float), KB = KP/4,
+  // LDS offset of the double-buffered coupling-channel array X
+  const float* dense_w;
+  int32_t dense_kb, dense_rb, lds_x;
   int64_t img_bytes;
   int32_t obs_sv[TVBK_KMAX_OBS];
   double obs_coef[TVBK_KMAX_OBS][8];
@@ -78,6 +82,7 @@ struct InitArgs {
 hipError_t launch_step(int precision, int model, int cf, const StepArgs& a, int ntiles, int nthreads, size_t lds,
                        hipStream_t st);
 hipError_t launch_init(int precision, int model, int cf, const InitArgs& a, int ntiles, hipStream_t st);
+hipError_t launch_dense(int model, const StepArgs& a, int ntiles, int nthreads, size_t lds, hipStream_t st);
 hipError_t launch_debug_slots(const uint32_t* edges, int nnz, int H, int64_t step, int32_t* out, hipStream_t st);
 // scatter a compact history [H][nch][N] / state [nsv][N] (plan precision)
 // into lane t of a tile, and the reverse gather.

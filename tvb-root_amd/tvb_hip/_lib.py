@@ -74,6 +74,8 @@ def _bind(lib):
         "tvbk_plan_destroy": ([P], C.c_int),
         "tvbk_plan_footprint": ([C.POINTER(PlanDesc), P], C.c_int),
         "tvbk_poll": ([P, P], C.c_int),
+        "tvbk_stream_create": ([P], C.c_int),
+        "tvbk_stream_destroy": ([P], C.c_int),
         "tvbk_set_history": ([P, C.c_int32, P, C.c_int32, P, C.c_int64], C.c_int),
         "tvbk_init_uniform": ([P, C.c_uint64, P, P], C.c_int),
         "tvbk_set_noise": ([P, P, C.c_int64], C.c_int),
@@ -99,7 +101,7 @@ def _bind(lib):
 
 EXPORTED = (
     "tvbk_abi_version", "tvbk_model_info", "tvbk_plan_create", "tvbk_plan_destroy",
-    "tvbk_plan_footprint", "tvbk_poll",
+    "tvbk_plan_footprint", "tvbk_poll", "tvbk_stream_create", "tvbk_stream_destroy",
     "tvbk_set_history", "tvbk_init_uniform", "tvbk_set_noise", "tvbk_run_steps",
     "tvbk_set_steps", "tvbk_sync", "tvbk_monitor_layout", "tvbk_read_monitor",
     "tvbk_monitor_device_ptr", "tvbk_export_state", "tvbk_read_flags",
@@ -113,6 +115,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise LibraryMissing("HIP library %s is not built (run __graft_entry__.build())" % LIB_PATH)
+    # PyTorch-ROCm ships its own HIP runtime: load it first when present so
+    # that this library binds to the same runtime (one device context per
+    # process, torch streams and pointers valid here).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     try:
         lib = C.CDLL(LIB_PATH)
     except OSError as exc:
@@ -138,6 +147,18 @@ def ptr(a):
         return None
     assert a.flags["C_CONTIGUOUS"]
     return a.ctypes.data
+
+
+def stream_create():
+    """A HIP stream owned by the caller (release with stream_destroy)."""
+    lib = load()
+    h = C.c_void_p()
+    check(lib.tvbk_stream_create(C.byref(h)))
+    return h.value
+
+
+def stream_destroy(stream):
+    check(load().tvbk_stream_destroy(stream))
 
 
 def model_info(model, cfun):

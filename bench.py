@@ -30,6 +30,22 @@ sys.path.insert(0, os.path.join(ROOT, "tvb-root_amd"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK_TFLOPS = 157.3
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_step_mpr_c2_hbm_traffic.md")
+
+
+def measured_traffic_per_region_step(path=TRAFFIC_SUMMARY):
+    """HBM bytes per region-step of the step kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (line
+    'traffic_bytes_per_region_step: X'), or None when there is none."""
+    try:
+        with open(path) as f:
+            for ln in f:
+                if ln.strip().startswith("traffic_bytes_per_region_step:"):
+                    return float(ln.split(":", 1)[1].split()[0])
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def log(*a):
@@ -127,14 +143,15 @@ def fp64_single_instance(w, d, G, sig, n_steps):
     return w.shape[0] * n_steps / el, el, shape
 
 
-def tavg_error_sample(w, L, d, G, sig, n_steps=4000, B=32, check=(0, 17)):
-    """FP32 batched kernel with injected noise vs the FP64 oracle on a
-    sample of the sweep: max |err| / max |ref| of TemporalAverage."""
+def tavg_error_sample(w, L, d, G, sig, precision, n_steps=4000, B=32, check=(0, 17)):
+    """The batched kernel (float32 history) with injected noise vs the FP64
+    oracle on a sample of the sweep, random initial history (its transient
+    included): max |err| / max |ref| of TemporalAverage."""
     from tvb_hip import _lib as Lb
     from oracle.tvb_oracle import OracleSim
     idx = np.linspace(0, len(G) - 1, B).astype(int)
     Gs, ss = G[idx], sig[idx]
-    plan = build_c2_plan(w, d, Gs, ss, 0.05, n_steps, Lb.F32, Lb.NOISE_INJECTED, 0, bold=0.0, tile=32)
+    plan = build_c2_plan(w, d, Gs, ss, 0.05, n_steps, precision, Lb.NOISE_INJECTED, 0, bold=0.0, tile=32)
     H = int(d.max()) + 1
     rng = np.random.default_rng(5)
     ic = np.empty((H, 2, w.shape[0]))
@@ -209,6 +226,63 @@ def run_c3(stream=None, n_inst=256, duration_scale=0.02):
     return info
 
 
+def run_c2_zero_delay(B=8192, n_steps=40000, chunk=10000, stream=None, warmup=1000, precision=None):
+    """C2 with infinite conduction speed: every delay zero, horizon 1, so
+    the coupling of a step is W . x(t) for all instances at once -- a
+    (68 x 68) x (68 x B) product per coupling variable, run on the dense
+    MFMA kernel.  Returns region-steps/s and the kernel's MFMA roofline
+    (2 N ncvar flops per region-step)."""
+    from tvb_hip import _lib as L
+    w, Lmat, d, H, G, sig = c2_setup(B)
+    dz = np.zeros_like(d)
+    prec = L.MIXED if precision is None else precision
+    plan = build_c2_plan(w, dz, G, sig, 0.05, max(chunk, warmup), prec, L.NOISE_PHILOX, 4321, stream=stream)
+    plan.init_uniform(98, [0.0, -2.0], [2.0, 1.5])
+    plan.run(warmup)
+    plan.sync()
+    t0 = time.perf_counter()
+    done, ms = 0, []
+    while done < n_steps:
+        n = min(chunk, n_steps - done)
+        plan.run(n)
+        ms.append((n, plan.last_run_ms()))
+        done += n
+    plan.sync()
+    el = time.perf_counter() - t0
+    full = [m for n, m in ms if n == chunk] or [m for _, m in ms]
+    med = float(np.median(full))
+    flops = 2.0 * 68 * 2 * 68 * B * chunk
+    tf = flops / (med * 1e-3) / 1e12
+    out = {"value": 68.0 * B * n_steps / el, "unit": "region-steps/s", "steps": n_steps, "instances": B,
+           "kernel": [plan.kernel, plan.tile, plan.waves],
+           "roofline": {"bound": "mfma", "achieved": tf, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": tf / FP32_MFMA_PEAK_TFLOPS, "traffic": None,
+                        "flops_per_region_step": 2.0 * 68 * 2, "median_launch_ms": med},
+           "nonfinite_instances": int((plan.flags() != 0).sum())}
+    plan.close()
+    return out
+
+
+def run_copy(nbytes=2 << 30, reps=10):
+    """Plain device copy of a known size (HBM read + write of nbytes each),
+    the calibration for FETCH_SIZE / WRITE_SIZE and the measured copy
+    bandwidth of the device."""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").uniform_()
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(reps):
+        dst.copy_(src)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / reps
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GB/s": 2.0 * nbytes / (ms * 1e-3) / 1e9,
+            "launches": reps + 1}
+
+
 def run_c4(n_requests=24, rate_hz=40.0, duration_scale=0.01, inst_range=(1, 64), seed=404):
     """Serving: Poisson stream of sweep requests (G2D or MPR, HeunStochastic,
     N log-uniform in [68, 998], instances in inst_range, durations
@@ -239,7 +313,9 @@ def main():
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=40000)
     ap.add_argument("--no-extras", action="store_true", help="skip CPU baseline, FP64 single path and error sample")
-    ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4"))
+    ap.add_argument("--workload", default="c2", choices=("c2", "c2z", "c3", "c4", "copy"))
+    ap.add_argument("--precision", default="mixed", choices=("f32", "mixed"),
+                    help="mixed: float32 history and coupling sums, float64 state (the reference's own mix)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: every instance of the sweep is independent and the "
@@ -255,6 +331,15 @@ def main():
         print(json.dumps({"metric": "region-steps/s (uneven JansenRit batch)", "workload": "c3",
                           **run_c3(stream)}), flush=True)
         return
+    if args.workload == "c2z":
+        print(json.dumps({"metric": "region-steps/s (zero-delay sweep, dense MFMA)", "workload": "c2z",
+                          **run_c2_zero_delay(args.instances, stream=stream,
+                                              precision=L.MIXED if args.precision == "mixed" else L.F32)}),
+                         flush=True)
+        return
+    if args.workload == "copy":
+        print(json.dumps({"metric": "device copy bandwidth", "workload": "copy", **run_copy()}), flush=True)
+        return
     if args.workload == "c4":
         print(json.dumps({"metric": "p50/p99 request latency (serving)", "workload": "c4", **run_c4()}),
               flush=True)
@@ -265,7 +350,8 @@ def main():
     nnz = int((w != 0).sum())
     log("workload c2: N=68 nnz=%d H=%d B=%d steps=%d" % (nnz, H, B, args.steps))
     chunk = min(args.chunk, max(args.steps, 1))
-    plan = build_c2_plan(w, d, G, sig, dt, max(chunk, args.warmup), L.F32, L.NOISE_PHILOX, 1234,
+    prec = L.MIXED if args.precision == "mixed" else L.F32
+    plan = build_c2_plan(w, d, G, sig, dt, max(chunk, args.warmup), prec, L.NOISE_PHILOX, 1234,
                          tile=args.tile, waves=args.waves, stream=stream)
     log("plan: kernel=%d tile=%d waves=%d lds=%d" % (plan.kernel, plan.tile, plan.waves, plan.lds_bytes))
     plan.init_uniform(99, [0.0, -2.0], [2.0, 1.5])
@@ -293,22 +379,29 @@ def main():
     # roofline of the step kernel
     full = [(n, ms) for n, ms in launches if n == chunk] or launches
     avg_ms = sum(ms for _, ms in full) / len(full)
+    med_ms = float(np.median([ms for _, ms in full]))
     bpr = algorithmic_bytes_per_region_step(nnz, 68, 2, 4, 2, 20, 2, 80, 5000, 40000)
     per_launch = bpr * 68 * B * full[0][0]
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    tpr = measured_traffic_per_region_step()
+    traffic = tpr * 68 * B * full[0][0] if tpr else None
     line = {
         "metric": "region-steps/s batched FP32 vs plain FP64 at stated error; p50/p99 latency",
         "value": value, "unit": "region-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": el * 1e3 / max(args.steps, 1), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f32 history/coupling, f64 state" if args.precision == "mixed" else "f32",
+        "data": "synthetic",
         "config": {"workload": "c2: MPR HeunStochastic dt=0.05 sweep, 68 nodes x %d instances, shared "
                                "25%%-density connectome (nnz %d, horizon %d), TemporalAverage 1 ms + Bold 2000 ms, "
                                "device Philox noise" % (B, nnz, H),
-                   "instances": B, "nodes": 68, "kernel_tile": plan.tile, "kernel_waves": plan.waves,
+                   "instances": B, "nodes": 68, "precision": args.precision, "kernel_tile": plan.tile,
+                   "kernel_waves": plan.waves,
                    "steps_per_launch": chunk},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes_per_region_step": bpr, "avg_launch_ms": avg_ms},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": per_launch, "algorithmic_bytes_per_region_step": bpr,
+                     "traffic_bytes_per_region_step": tpr, "avg_launch_ms": avg_ms, "median_launch_ms": med_ms,
+                     "launches": len(full)},
         "nonfinite_instances": int((flags != 0).sum()),
     }
     plan.close()
@@ -317,11 +410,12 @@ def main():
         line["fp64_single_instance"] = {"value": v64, "unit": "region-steps/s", "steps": 40000,
                                         "kernel": list(shape64)}
         line["batched_fp32_over_fp64_single"] = value / v64
-        line["tavg_max_rel_err_vs_fp64_oracle"] = tavg_error_sample(w, Lmat, d, G, sig)
+        line["tavg_max_rel_err_vs_fp64_oracle"] = tavg_error_sample(w, Lmat, d, G, sig, prec)
         line["serving"] = run_c4()
         line["p50_latency_s"] = line["serving"]["p50_latency_s"]
         line["p99_latency_s"] = line["serving"]["p99_latency_s"]
         line["uneven_batch"] = run_c3(stream)
+        line["zero_delay_mfma"] = run_c2_zero_delay(B, stream=stream, precision=prec)
         cpu_rate, cpu_el = cpu_baseline(w, Lmat, 1.0, 1e-3, args.cpu_steps)
         line["cpu_baseline"] = {"value": cpu_rate, "unit": "region-steps/s", "cores": 1, "kind": "port",
                                 "sample": "1 instance x %d steps (%.1f s simulated) of the c2 workload, "

@@ -69,7 +69,15 @@ enum {
   TVBK_HEUN_STO = 3, TVBK_IDENTITY = 4
 };
 
-enum { TVBK_F32 = 0, TVBK_F64 = 1 };
+/* Precision of a plan.
+ *   F32    float32 history, weights, coupling sums, state and monitors.
+ *   F64    float64 throughout.
+ *   MIXED  the reference's own mix: float32 history ring, nnz weights and
+ *          Linear sums (history.py:49-51, :218; coupling.py:166), float64
+ *          coupling post, state, integration and monitors (Raw, TAvg,
+ *          Bold outputs float64).  Not with SIGMOID_JR, whose reference
+ *          pre/sum run in float64 on float32 history values. */
+enum { TVBK_F32 = 0, TVBK_F64 = 1, TVBK_MIXED = 2 };
 
 /* Noise source of stochastic schemes.
  *   INJECTED  caller-supplied standard normals z[step][svar][node][inst]

@@ -113,3 +113,61 @@ def test_mpr_fp32_vs_fp64_oracle():
     assert e_t <= 1e-4
     assert e_b <= 1e-6
     assert per_region_corr(ta_d, ta_r) >= 0.999
+
+
+@pytest.mark.parametrize("model,scheme,cparams,nsig,dt,exact", [
+    ("Generic2dOscillator", "HeunDeterministic", dict(a=0.0126), None, 0.1, True),
+    ("MontbrioPazoRoxin", "HeunStochastic", dict(a=0.5), [0.01, 0.02], 0.05, True),
+    ("MontbrioPazoRoxin", "EulerStochastic", dict(a=0.3), [0.01, 0.01], 0.05, True),
+    ("Generic2dOscillator", "EulerDeterministic", dict(a=0.02), None, 0.1, True),
+    ("JansenRit", "EulerStochastic", dict(a=1e-3), [0.0, 0.0, 0.0, 0.01, 0.01, 0.01], 0.1, False),
+])
+def test_mixed_matches_faithful_oracle(model, scheme, cparams, nsig, dt, exact):
+    """MIXED device mode (float32 history, weights and Linear sums; float64
+    post, state, integration and monitors -- the reference's own precision)
+    vs the oracle's reference-faithful mode, injected noise.  G2D and MPR
+    (polynomial dfuns): bit-identical Raw, TemporalAverage, final state and
+    float32 history; Bold max rel <= 1e-13 (its 5000-tap sum runs in another
+    order than numpy.dot's).  JansenRit (exp in its sigmoid, whose last bit
+    may differ between libraries): max rel <= 1e-10."""
+    _say("mixed vs faithful", model, scheme)
+    mons = (("Raw", None, None), ("TemporalAverage", 1.0, None), ("Bold", 20.0, None))
+    orc, sim, _ = make_case(model, n=20, density=0.3, lengths=(5.0, 60.0), speed=3.0, dt=dt,
+                            scheme=scheme, cparams=cparams, nsig=nsig, monitors=mons,
+                            spread=0.02 if model == "JansenRit" else 1.0)
+    nstep = 1500
+    o = orc["faithful"]
+    z = np.random.default_rng(11).standard_normal((nstep, len(sim.model.state_variables), o.N))
+    ref = o.run(nstep, noise=z if nsig is not None else None)
+    out = _backend().run_sim(sim, nstep=nstep, precision="mixed", noise=z)
+    for (tr, xr), (td, xd), name in zip(ref, out, ("raw", "tavg", "bold")):
+        assert xd.shape == xr.shape, (name, xd.shape, xr.shape)
+        assert xd.dtype == np.float64
+        np.testing.assert_allclose(td, tr, rtol=0, atol=1e-9)
+        e = rel_err(xd, xr)
+        _say("  ", name, "rel", e)
+        if exact and name != "bold":
+            np.testing.assert_array_equal(xd, xr)
+        else:
+            assert e <= (1e-13 if exact else 1e-10), (name, e)
+    # the written-back simulator state is the oracle's: float64 state and
+    # the float32 history ring in the reference's slot order
+    st = np.asarray(sim.current_state)[..., 0]
+    hb = np.asarray(sim.history.buffer)[..., 0]
+    assert hb.dtype == np.float32
+    if exact:
+        np.testing.assert_array_equal(st, o.current_state[..., 0])
+        np.testing.assert_array_equal(hb, o.ring)
+    else:
+        assert rel_err(st, o.current_state[..., 0]) <= 1e-10
+        assert rel_err(hb, o.ring) <= 1e-6
+
+
+def test_mixed_rejects_sigmoidal_coupling():
+    """MIXED with SigmoidalJansenRit is refused (its reference pre and sum
+    run in float64 on float32 history values, which the ring does not hold)."""
+    import tvb_hip._lib as L
+    orc, sim, _ = make_case("JansenRit", n=8, scheme="EulerStochastic", coupling="SigmoidalJansenRit",
+                            nsig=[0.0, 0.0, 0.0, 0.01, 0.01, 0.01], spread=0.02)
+    with pytest.raises(L.TvbkError):
+        _backend().build_plan(sim, precision="mixed", max_steps=4)

@@ -112,7 +112,8 @@ struct tvbk_plan {
   ModelInfo mi;
   int nch = 0, nct = 0, ncp = 0, prow = 0;
   int kernel = 0, T = 1, waves = 1, nthreads = 64;
-  int esize = 4;
+  int esize = 4;    // state, parameters, Raw/TAvg outputs
+  int hsize = 4;    // history ring, weights, coupling sums
   bool stochastic = false;
   hipStream_t stream = nullptr;
   std::string err;
@@ -252,12 +253,13 @@ static int layout_lds(tvbk_plan* p, int T, int64_t max_lds, bool allow_csr) {
     if (d.mon[m].kind == TVBK_MON_TAVG) {
       off = align_up(off, 8);
       p->mon_lds_off[m] = (int)off;
-      off += (int64_t)N * d.mon[m].n_voi * 2 * T * p->esize;
+      // float32: sum and Kahan compensation; float64: sum
+      off += (int64_t)N * d.mon[m].n_voi * (p->esize == 8 ? 1 : 2) * T * p->esize;
     }
   off = align_up(off, 16);
   p->img_bytes = off;
   p->lds_c = (int)off;
-  off += 2 * (int64_t)N * p->nct * T * p->esize;
+  off += 2 * (int64_t)N * p->nct * T * p->hsize;
   off = align_up(off, 16);
   // CSR staged as 16-byte records {col offset, delay, weight bits}
   const int64_t ne = p->max_nnz + TVBK_ROW_PAD;
@@ -298,8 +300,16 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   p->stochastic = d.scheme == TVBK_EULER_STO || d.scheme == TVBK_HEUN_STO;
   if (p->stochastic && d.noise == TVBK_NOISE_NONE) { p->err = "stochastic scheme needs a noise source"; return bail(TVBK_EINVAL); }
   if (!p->stochastic && d.noise != TVBK_NOISE_NONE) { p->err = "deterministic scheme takes no noise"; return bail(TVBK_EINVAL); }
-  if (d.precision != TVBK_F32 && d.precision != TVBK_F64) { p->err = "bad precision"; return bail(TVBK_EINVAL); }
-  p->esize = d.precision == TVBK_F64 ? 8 : 4;
+  if (d.precision != TVBK_F32 && d.precision != TVBK_F64 && d.precision != TVBK_MIXED) {
+    p->err = "bad precision";
+    return bail(TVBK_EINVAL);
+  }
+  if (d.precision == TVBK_MIXED && d.cfun == TVBK_CFUN_SIGMOID_JR) {
+    p->err = "mixed precision is not offered with SigmoidalJansenRit (its reference pre and sum run in float64)";
+    return bail(TVBK_EUNSUPPORTED);
+  }
+  p->esize = d.precision == TVBK_F32 ? 4 : 8;
+  p->hsize = d.precision == TVBK_F64 ? 8 : 4;
   if (d.n_inst <= 0 || d.n_conn <= 0 || !d.conns || !d.params) { p->err = "empty plan"; return bail(TVBK_EINVAL); }
   const int nparam_expect = p->mi.nmp + p->ncp + (p->stochastic ? p->mi.nsv : 0);
   if (d.n_param != nparam_expect) {
@@ -390,7 +400,7 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   int kernel = d.kernel;
   // dense zero-delay kernel: every delay zero (horizon 1), one shared
   // connectome of at most 128 nodes, Linear coupling, float32, G2D / MPR
-  const bool dense_ok = shared && d.precision == TVBK_F32 && d.cfun == TVBK_CFUN_LINEAR &&
+  const bool dense_ok = shared && d.precision != TVBK_F64 && d.cfun == TVBK_CFUN_LINEAR &&
                         (d.model == TVBK_MODEL_G2D || d.model == TVBK_MODEL_MPR) && p->conns.size() == 1 &&
                         p->conns[0].horizon == 1 && p->conns[0].n_node <= 128;
   if (kernel == TVBK_KERNEL_AUTO) {
@@ -398,7 +408,7 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
     else kernel = (shared && d.n_inst >= 32) ? TVBK_KERNEL_BATCH_LANE : TVBK_KERNEL_EDGE_LANE;
   }
   if (kernel == TVBK_KERNEL_DENSE_MFMA && !dense_ok) {
-    p->err = "dense MFMA kernel needs float32, Linear coupling, G2D or MPR, one connectome of <= 128 nodes "
+    p->err = "dense MFMA kernel needs a float32 history, Linear coupling, G2D or MPR, one connectome of <= 128 nodes "
              "with all delays zero";
     return bail(TVBK_EUNSUPPORTED);
   }
@@ -426,10 +436,19 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
     }
     if (T != 16 && T != 32 && T != 64) { p->err = "tile must be 16, 32 or 64"; return bail(TVBK_EINVAL); }
     while (layout_lds(p, T, max_lds, true) != TVBK_OK) {
-      if (d.tile || T == 16) { p->err = "tile does not fit in LDS"; return bail(TVBK_EUNSUPPORTED); }
+      if (d.tile || T == 16) {
+        if (d.kernel == TVBK_KERNEL_AUTO && !d.tile) {   // large nodes: one instance per workgroup
+          kernel = p->kernel = TVBK_KERNEL_EDGE_LANE;
+          T = 1;
+          break;
+        }
+        p->err = "tile does not fit in LDS";
+        return bail(TVBK_EUNSUPPORTED);
+      }
       T /= 2;
     }
-  } else {
+  }
+  if (kernel == TVBK_KERNEL_EDGE_LANE) {
     if (d.tile && d.tile != 1) { p->err = "edge-lane kernel uses tile 1"; return bail(TVBK_EINVAL); }
     if (layout_lds(p, 1, max_lds, true) != TVBK_OK) { p->err = "node state does not fit in LDS"; return bail(TVBK_EUNSUPPORTED); }
   }
@@ -441,7 +460,7 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
     else if (d.waves) nw = d.waves;
     else if (T > 1) nw = std::min(16, std::max(1, (P + 64 / T - 1) / (64 / T)));
     else nw = std::min(16, std::max(1, (std::max(P, p->max_n) + 63) / 64));
-    const int max_waves = p->esize == 8 ? 8 : 16;   // the float64 kernels are built for 512 threads
+    const int max_waves = p->hsize == 8 ? 8 : 16;   // the float64-history kernels are built for 512 threads
     if (!d.waves) nw = std::min(nw, max_waves);
     if (nw < 1 || nw > max_waves) {
       p->err = "waves must be in [1, " + std::to_string(max_waves) + "]";
@@ -485,11 +504,11 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
     }
     td.ring_off = ring_off;
     ring_off += (int64_t)cd.horizon * cd.n_node * p->nch * T;
-    if ((int64_t)cd.horizon * cd.n_node * p->nch * T * p->esize >= (1LL << 31)) {
+    if ((int64_t)cd.horizon * cd.n_node * p->nch * T * p->hsize >= (1LL << 31)) {
       p->err = "tile history exceeds 2 GiB";
       return bail(TVBK_EUNSUPPORTED);
     }
-    if ((int64_t)cd.n_node * p->nch * T * p->esize >= (1LL << 24)) {
+    if ((int64_t)cd.n_node * p->nch * T * p->hsize >= (1LL << 24)) {
       p->err = "one history slot of a tile exceeds 16 MiB";
       return bail(TVBK_EUNSUPPORTED);
     }
@@ -525,8 +544,8 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   // ---- device footprint (dry run stops here)
   {
     size_t tot = p->conns.size() * sizeof(ConnDesc) + edges.size() * 4 + 16 + rp.size() * 4 +
-                 (w64.size() * p->esize + 16) + prm.size() * p->esize + (d.inst_stream ? (size_t)B * 8 : 0) +
-                 (size_t)ring_off * p->esize + (size_t)img_off + (size_t)bacc_off * 8 + (size_t)B * 4 +
+                 (w64.size() * p->hsize + 16) + prm.size() * p->esize + (d.inst_stream ? (size_t)B * 8 : 0) +
+                 (size_t)ring_off * p->hsize + (size_t)img_off + (size_t)bacc_off * 8 + (size_t)B * 4 +
                  p->tiles.size() * sizeof(TileDesc);
     for (int m = 0; m < d.n_mon; ++m) {
       if (d.mon[m].kind == TVBK_MON_BOLD) tot += p->taps_host[m].size() * 8;
@@ -563,13 +582,16 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   if ((rc = up(&p->d_edges, edges.data(), edges.size() * 4))) return bail(rc);
   if ((rc = up(&p->d_rp, rp.data(), rp.size() * 4))) return bail(rc);
   std::vector<float> w32, p32, wd;   // host sources of async uploads: alive until the final sync
-  if (p->esize == 8) {
+  if (p->hsize == 8) {
     if ((rc = up(&p->d_w, w64.data(), w64.size() * 8))) return bail(rc);
-    if ((rc = up(&p->d_params, prm.data(), prm.size() * 8))) return bail(rc);
   } else {
     w32.assign(w64.begin(), w64.end());
-    p32.assign(prm.begin(), prm.end());
     if ((rc = up(&p->d_w, w32.data(), w32.size() * 4))) return bail(rc);
+  }
+  if (p->esize == 8) {
+    if ((rc = up(&p->d_params, prm.data(), prm.size() * 8))) return bail(rc);
+  } else {
+    p32.assign(prm.begin(), prm.end());
     if ((rc = up(&p->d_params, p32.data(), p32.size() * 4))) return bail(rc);
   }
   if (d.inst_stream) {
@@ -586,7 +608,7 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   for (int m = 0; m < d.n_mon; ++m)
     if (d.mon[m].kind == TVBK_MON_BOLD)
       if ((rc = up(&p->d_taps[m], p->taps_host[m].data(), p->taps_host[m].size() * 8))) return bail(rc);
-  if ((rc = dev_alloc(p, &p->d_ring, (size_t)ring_off * p->esize))) return bail(rc);
+  if ((rc = dev_alloc(p, &p->d_ring, (size_t)ring_off * p->hsize))) return bail(rc);
   if ((rc = dev_alloc(p, &p->d_img, (size_t)img_off))) return bail(rc);
   if ((rc = dev_alloc(p, &p->d_bacc, (size_t)bacc_off * 8))) return bail(rc);
   if ((rc = dev_alloc(p, &p->d_flags, (size_t)B * 4))) return bail(rc);
@@ -675,15 +697,16 @@ int tvbk_set_history(tvbk_plan* p, int32_t inst, const void* history, int32_t el
   const ConnDesc& cd = p->conns[conn];
   const int H = cd.horizon, N = cd.n_node, nch = p->nch, nsv = p->mi.nsv;
   const size_t nh = (size_t)H * nch * N;
-  // stage compact arrays in plan precision on the device
-  std::vector<unsigned char> hbuf(nh * p->esize), sbuf((size_t)nsv * N * p->esize);
+  // stage compact arrays on the device: history in the ring's precision,
+  // state in the state precision
+  std::vector<unsigned char> hbuf(nh * p->hsize), sbuf((size_t)nsv * N * p->esize);
   std::vector<unsigned char> hin(nh * elem_bytes);
   HIPCHK(p, hipMemcpy(hin.data(), history, hin.size(), hipMemcpyDefault));
   std::vector<double> sin((size_t)nsv * N);
   HIPCHK(p, hipMemcpy(sin.data(), state, sin.size() * 8, hipMemcpyDefault));
   for (size_t k = 0; k < nh; ++k) {
     const double v = elem_bytes == 8 ? reinterpret_cast<double*>(hin.data())[k] : reinterpret_cast<float*>(hin.data())[k];
-    if (p->esize == 8) reinterpret_cast<double*>(hbuf.data())[k] = v;
+    if (p->hsize == 8) reinterpret_cast<double*>(hbuf.data())[k] = v;
     else reinterpret_cast<float*>(hbuf.data())[k] = (float)v;
   }
   for (size_t k = 0; k < sin.size(); ++k) {
@@ -698,7 +721,7 @@ int tvbk_set_history(tvbk_plan* p, int32_t inst, const void* history, int32_t el
   hipError_t e = hipSuccess;
   for (int b = b0; b < b1 && e == hipSuccess; ++b) {
     const TileDesc& td = p->tiles[p->tile_of[b]];
-    e = launch_scatter(p->d.precision, false, (char*)p->d_ring.p + td.ring_off * p->esize,
+    e = launch_scatter(p->d.precision, false, (char*)p->d_ring.p + td.ring_off * p->hsize,
                        (unsigned char*)p->d_img.p + td.img_off + p->lds_state, dh, ds, H, N, nch, nsv, p->T,
                        p->lane_of[b], p->stream);
     p->cur_step[b] = current_step;
@@ -720,12 +743,12 @@ int tvbk_export_state(tvbk_plan* p, int32_t inst, void* history, int32_t elem_by
   const int H = cd.horizon, N = cd.n_node, nch = p->nch, nsv = p->mi.nsv;
   const size_t nh = (size_t)H * nch * N;
   void *dh = nullptr, *ds = nullptr;
-  HIPCHK(p, hipMalloc(&dh, nh * p->esize));
+  HIPCHK(p, hipMalloc(&dh, nh * p->hsize));
   HIPCHK(p, hipMalloc(&ds, (size_t)nsv * N * p->esize));
-  hipError_t e = launch_scatter(p->d.precision, true, (char*)p->d_ring.p + td.ring_off * p->esize,
+  hipError_t e = launch_scatter(p->d.precision, true, (char*)p->d_ring.p + td.ring_off * p->hsize,
                                 (unsigned char*)p->d_img.p + td.img_off + p->lds_state, dh, ds, H, N, nch, nsv, p->T,
                                 p->lane_of[inst], p->stream);
-  std::vector<unsigned char> hb(nh * p->esize), sb((size_t)nsv * N * p->esize);
+  std::vector<unsigned char> hb(nh * p->hsize), sb((size_t)nsv * N * p->esize);
   if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
   if (e == hipSuccess) e = hipMemcpy(hb.data(), dh, hb.size(), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(sb.data(), ds, sb.size(), hipMemcpyDeviceToHost);
@@ -735,7 +758,7 @@ int tvbk_export_state(tvbk_plan* p, int32_t inst, void* history, int32_t elem_by
   if (history) {
     std::vector<unsigned char> ho(nh * elem_bytes);
     for (size_t k = 0; k < nh; ++k) {
-      const double v = p->esize == 8 ? reinterpret_cast<double*>(hb.data())[k] : reinterpret_cast<float*>(hb.data())[k];
+      const double v = p->hsize == 8 ? reinterpret_cast<double*>(hb.data())[k] : reinterpret_cast<float*>(hb.data())[k];
       if (elem_bytes == 8) reinterpret_cast<double*>(ho.data())[k] = v;
       else reinterpret_cast<float*>(ho.data())[k] = (float)v;
     }
@@ -934,7 +957,8 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
   a.lds_x = p->lds_x;
   HIPCHK(p, hipEventRecord(p->ev0, p->stream));
   if (p->kernel == TVBK_KERNEL_DENSE_MFMA)
-    HIPCHK(p, launch_dense(d.model, a, (int)p->tiles.size(), p->nthreads, (size_t)p->lds_total, p->stream));
+    HIPCHK(p, launch_dense(d.precision, d.model, a, (int)p->tiles.size(), p->nthreads, (size_t)p->lds_total,
+                           p->stream));
   else
     HIPCHK(p, launch_step(d.precision, d.model, d.cfun, a, (int)p->tiles.size(), p->nthreads, (size_t)p->lds_total,
                           p->stream));

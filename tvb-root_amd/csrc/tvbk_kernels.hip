@@ -1,627 +1,27 @@
-// Persistent fused step kernel for TVB's simulator loop on gfx950.
-//
-// One workgroup owns a "tile": T simulation instances that share one
-// connectome (T = 16/32/64 for parameter sweeps over a shared connectome;
-// T = 1 for instances with their own connectome, one per workgroup).  The
-// workgroup advances its tile through all steps of a call without any
-// inter-workgroup synchronisation: instances are independent.
-//
-// Per step s (Simulator.__call__, simulator.py:356-363, with the coupling of
-// step s evaluated from the history as it stood after step s-1):
-//   Phase A  coupling c(s): every (target node, history channel) pair is
-//            reduced by a group of T lanes (one lane per instance) over the
-//            node's CSR row in numpy add.reduceat order, gathering
-//            history[(s-1-d) % H][source][channel][lane] -- T consecutive
-//            values per edge, so each gather is one coalesced segment.
-//   barrier
-//   Phase B  integrator step of every (node, instance) with c(s), write of
-//            history slot s % H, on-device monitors.
-// With every delay >= 1, phase A of step s+1 only reads slots written before
-// the barrier of step s, so one barrier per step suffices; coupling terms are
-// double-buffered by step parity.  With zero delays a second barrier orders
-// the history write before the next gather.
-//
-// Layouts (elements of the plan precision R unless stated):
-//   history  ring[tile][slot][node][channel][T]
-//   LDS      bold interim (f64) | state [node][svar][T] | tavg sum, comp
-//            [node][voi][T] | c[2][node][term][T] | CSR (packed col|delay,
-//            weight, row_ptr) when it fits
-//   the persistent part of LDS (bold interim, state, tavg) is copied from and
-//   back to a per-tile image in HBM around each call.
+// Small kernels (history scatter/gather, slot arithmetic check) and the
+// dispatch from the plan's runtime (precision, model, coupling) to the
+// instantiated step, init and dense kernels of tvbk_inst.hip.
 #include "tvbk_kernels.hpp"
 #include "tvbk_device.hpp"
 
 namespace tvbk {
 
-template <class R>
-__device__ inline void copy_bytes(unsigned char* dst, const unsigned char* src, int64_t n) {
-  // 16-byte granularity copy; all images are padded to 16 bytes.
-  const int64_t n16 = n >> 4;
-  uint4* d = reinterpret_cast<uint4*>(dst);
-  const uint4* s = reinterpret_cast<const uint4*>(src);
-  for (int64_t k = threadIdx.x; k < n16; k += blockDim.x) d[k] = s[k];
-}
+template <class RH, class R, class M, int CF>
+hipError_t launch_step_t(const StepArgs& a, int ntiles, int nthreads, size_t lds, hipStream_t st);
+template <class RH, class R, class M, int CF>
+hipError_t launch_init_t(const InitArgs& a, int ntiles, hipStream_t st);
+template <class R, class M>
+hipError_t launch_dense_t(const StepArgs& a, int ntiles, int nthreads, size_t lds, hipStream_t st);
 
-// Tile, monitor and observable configuration staged in LDS once per launch,
-// plus the per-step monitor bookkeeping (double-buffered by step parity,
-// written by thread 0 for the next step before it reaches that step's
-// barrier, read by everyone in phase B).
-struct StepInfo {
-  int32_t emit, push, msample, pad;
-  int64_t mp, kk;
-};
-
-struct KCfg {
-  TileDesc td;
-  MonArgs mon[TVBK_KMAX_MON];
-  StepInfo si[2][TVBK_KMAX_MON];
-  int32_t obs_sv[TVBK_KMAX_OBS];
-  double obs_coef[TVBK_KMAX_OBS][8];
-};
-
-// One CSR entry as staged in LDS: byte offset of the source node inside a
-// history slot, delay, and the weight's bits.
-struct EdgeRec {
-  uint32_t coff, delay, w0, w1;
-};
-
-template <class R> __device__ inline R rec_weight(const EdgeRec& e);
-template <> __device__ inline float rec_weight<float>(const EdgeRec& e) { return __uint_as_float(e.w0); }
-template <> __device__ inline double rec_weight<double>(const EdgeRec& e) {
-  return __hiloint2double((int)e.w1, (int)e.w0);
-}
-
-// Per-step monitor bookkeeping for step s = s0 + k (uniform): emission when
-// s % istep == 0 (sample index within the call), Bold push when s % ii == 0
-// (interim sample index mp = s / ii) and emission index kk = s / istep.
-__device__ inline void step_info(const MonArgs& mo, int k, StepInfo& si) {
-  const int ek = mo.r0 + k;
-  si.emit = (ek % mo.istep) == 0;
-  si.msample = ek / mo.istep - (mo.r0 > 0 ? 1 : 0);
-  si.kk = mo.qe0 + ek / mo.istep;
-  const int pk = mo.rb0 + k;
-  si.push = (pk % mo.ii) == 0;
-  si.mp = mo.qb0 + pk / mo.ii;
-}
-
-// Integrator step, history write and monitors of one (node i, lane t) pair
-// with its coupling terms c (phase B of the step kernels).  `sink(ch, v)`
-// stores history channel ch of the new state.
-template <class R, class M, int CF, class SINK>
-__device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, unsigned char* smem, R* s_x, int i,
-                                            int t, int T, int N, bool live, int inst, uint64_t stream, int k,
-                                            int64_t s, int buf, const R* pm, const R* cp, R* gf, const R* c,
-                                            int& bad, SINK sink) {
-  using C = Cfun<M, CF>;
-  constexpr int NSV = M::NSV, NDP = M::NDP, NCP = C::NCP, NCH = C::NCH;
-  const TileDesc& td = kc.td;
-  const R dt = (R)a.dt;
-  const int64_t spatial_row = (int64_t)N * a.prow;
-  R x[NSV], xi[NSV];
-#pragma unroll
-  for (int v = 0; v < NSV; ++v) x[v] = s_x[(i * NSV + v) * T + t];
-  R pp[NDP > 0 ? NDP : 1];
-#pragma unroll
-  for (int v = 0; v < NDP; ++v) pp[v] = pm[v];
-  if (a.spatial) {
-    const R* pr = reinterpret_cast<const R*>(a.params) + (int64_t)inst * spatial_row + (int64_t)i * a.prow;
-#pragma unroll
-    for (int v = 0; v < NDP; ++v) pp[v] = pr[v];
-#pragma unroll
-    for (int v = 0; v < NSV; ++v) gf[v] = (a.noise != NOISE_NONE) ? pr[NDP + NCP + v] : R(0);
-  }
-  if (a.noise == NOISE_INJECTED) {
-    // padded lanes (t >= n_inst) read lane 0's noise: never past the array
-    const double* z = a.z + td.z_off + k * td.z_sk + (int64_t)i * td.z_si + (live ? t : 0);
-#pragma unroll
-    for (int v = 0; v < NSV; ++v) xi[v] = (R)((a.sqrt_dt * z[v * td.z_ssv]) * (double)gf[v]);
-  } else if (a.noise == NOISE_PHILOX) {
-    R zz[NSV];
-    philox_normals<R>(NSV, (uint32_t)i, stream, s, a.key0, a.key1, zz);
-#pragma unroll
-    for (int v = 0; v < NSV; ++v) xi[v] = ((R)a.sqrt_dt * zz[v]) * gf[v];
-  } else {
-#pragma unroll
-    for (int v = 0; v < NSV; ++v) xi[v] = R(0);
-  }
-  scheme_step<R, M>(a.scheme, x, c, pp, dt, xi);
-#pragma unroll
-  for (int v = 0; v < NSV; ++v) {
-    bad |= !isfinite(x[v]);
-    s_x[(i * NSV + v) * T + t] = x[v];
-  }
-#pragma unroll
-  for (int ch = 0; ch < NCH; ++ch) sink(ch, C::template channel<R>(x, ch, cp));
-
-  // ---------------- monitors (monitors.py:160-190, :363-391, :834-962)
-  for (int m = 0; m < a.n_mon; ++m) {
-    const MonArgs& mo = kc.mon[m];
-    const StepInfo& si = kc.si[buf][m];
-    const int nv = mo.nvoi;
-    const bool emit = si.emit != 0;
-    if (mo.kind == MON_RAW || mo.kind == MON_TAVG) {
-      R* out = reinterpret_cast<R*>(a.out[m]) + td.out_off[m] + (int64_t)si.msample * td.out_sj[m] +
-               (int64_t)i * td.out_si[m] + t;
-      R* ts = reinterpret_cast<R*>(smem + mo.lds_off);
-      for (int v = 0; v < nv; ++v) {
-        const int o = mo.voi[v];
-        R val;
-        const int osv = kc.obs_sv[o];
-        if (osv >= 0) {
-          val = x[0];
-#pragma unroll
-          for (int u = 1; u < NSV; ++u) val = (u == osv) ? x[u] : val;
-        } else {
-          val = R(0);
-#pragma unroll
-          for (int u = 0; u < NSV; ++u) val = val + (R)kc.obs_coef[o][u] * x[u];
-        }
-        if (mo.kind == MON_RAW) {
-          if (live) out[v * td.out_sv[m]] = val;
-        } else {
-          // running sum over the averaging window: Kahan-compensated in
-          // float32, plain sequential (numpy's mean order) in float64
-          R* sum = ts + ((i * nv + v) * 2 + 0) * T + t;
-          R* cmp = ts + ((i * nv + v) * 2 + 1) * T + t;
-          R tv;
-          if constexpr (sizeof(R) == 8) {
-            tv = *sum + val;
-          } else {
-            const R yv = val - *cmp;
-            tv = *sum + yv;
-            *cmp = (tv - *sum) - yv;
-          }
-          *sum = tv;
-          if (emit) {
-            if (live) out[v * td.out_sv[m]] = tv / (R)mo.istep;
-            *sum = R(0);
-            *cmp = R(0);
-          }
-        }
-      }
-    } else {  // Bold: interim mean every ii steps pushed into FIR accumulators
-      double* bi = reinterpret_cast<double*>(smem + mo.lds_off);
-      const bool push = si.push != 0;
-      const int64_t mp = si.mp;                      // interim sample index s / ii
-      double* acc = a.bacc + td.bacc_off[m];
-      for (int v = 0; v < nv; ++v) {
-        const int o = mo.voi[v];
-        R val;
-        const int osv = kc.obs_sv[o];
-        if (osv >= 0) {
-          val = x[0];
-#pragma unroll
-          for (int u = 1; u < NSV; ++u) val = (u == osv) ? x[u] : val;
-        } else {
-          val = R(0);
-#pragma unroll
-          for (int u = 0; u < NSV; ++u) val = val + (R)kc.obs_coef[o][u] * x[u];
-        }
-        double* bsum = bi + (i * nv + v) * T + t;
-        *bsum = *bsum + (double)val;
-        if (push) {
-          const double mean = *bsum / (double)mo.ii;
-          *bsum = 0.0;
-          // sample mp contributes to emissions kk with kk*istep >= mp*ii
-          // and kk*istep < (mp+S)*ii; weight by age a = floor(kk*istep/ii) - mp:
-          // a == 0 -> G[S-1], else G[a-1]
-          const int64_t klo = (mp * mo.ii + mo.istep - 1) / mo.istep;
-          const int64_t khi = ((mp + mo.S) * mo.ii - 1) / mo.istep;
-          for (int64_t kk = klo; kk <= khi; ++kk) {
-            const int64_t age = (kk * mo.istep) / mo.ii - mp;
-            const double w = mo.taps[age == 0 ? mo.S - 1 : age - 1];
-            double* ap = acc + (((kk % mo.Q) * nv + v) * (int64_t)N + i) * T + t;
-            *ap = *ap + w * mean;
-          }
-        }
-        if (emit) {
-          const int64_t kk = si.kk;                  // s / istep
-          double* ap = acc + (((kk % mo.Q) * nv + v) * (int64_t)N + i) * T + t;
-          double* out = reinterpret_cast<double*>(a.out[m]) + td.out_off[m] +
-                        (int64_t)si.msample * td.out_sj[m] + (int64_t)v * td.out_sv[m] +
-                        (int64_t)i * td.out_si[m] + t;
-          const double bold = mo.sub1 ? (*ap - 1.0) * mo.k1v0 : *ap;
-          if (live) *out = bold;
-          *ap = 0.0;
-        }
-      }
-    }
-  }
-}
-
-// float64 instances get up to 8 waves per workgroup and twice the registers
-template <class R, class M, int CF>
-__global__ void __launch_bounds__(sizeof(R) == 8 ? 512 : 1024) k_step(StepArgs a) {
-  using C = Cfun<M, CF>;
-  constexpr int NSV = M::NSV, NCH = C::NCH, NCT = C::NCT, NDP = M::NDP, NCP = C::NCP;
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ KCfg kc;
-
-  if (a.tiles[blockIdx.x].n_steps <= 0) return;   // uniform: whole block leaves
-  // scalars of this tile and its connectome: loaded straight from global
-  // memory (uniform addresses -> scalar loads, values stay in SGPRs)
-  const TileDesc* tdg = a.tiles + blockIdx.x;
-  const int conn = tdg->conn;
-  const ConnDesc* cdg = a.conns + conn;
-  const int N = cdg->n_node, H = cdg->horizon;
-  const int cd_nnz = cdg->nnz, cd_max_row = cdg->max_row;
-  const int64_t cd_e_off = cdg->e_off, cd_rp_off = cdg->rp_off;
-  const int td_inst0 = tdg->inst0, td_n_inst = tdg->n_inst;
-  const int64_t td_step0 = tdg->step0, td_n_steps = tdg->n_steps, td_ring_off = tdg->ring_off,
-                td_img_off = tdg->img_off;
-  const int T = a.T;
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const int lane = tid & 63, wave = tid >> 6, nwave = nthr >> 6;
-  const int nsteps = (int)td_n_steps;
-  const int rstride = NCH * T;                                   // elements per (slot, node)
-  const uint32_t cstride = (uint32_t)rstride * (uint32_t)sizeof(R);   // bytes per node in a slot
-  const uint32_t sstride = (uint32_t)N * cstride;                // bytes per slot (< 2^24)
-
-  if (tid == 0) {
-    kc.td = *tdg;
-    for (int m = 0; m < TVBK_KMAX_MON; ++m) {
-      MonArgs mo = a.mon[m];
-      if (m < a.n_mon) {
-        mo.r0 = (int)(td_step0 % mo.istep);
-        mo.qe0 = td_step0 / mo.istep;
-        mo.rb0 = (int)(td_step0 % mo.ii);
-        mo.qb0 = td_step0 / mo.ii;
-        step_info(mo, 0, kc.si[0][m]);
-      }
-      kc.mon[m] = mo;
-    }
-    for (int o = 0; o < TVBK_KMAX_OBS; ++o) {
-      kc.obs_sv[o] = a.obs_sv[o];
-      for (int u = 0; u < 8; ++u) kc.obs_coef[o][u] = a.obs_coef[o][u];
-    }
-  }
-  const TileDesc& td = kc.td;
-
-  // ---- LDS carve ---------------------------------------------------------
-  unsigned char* img = smem;                              // persistent image
-  R* s_x = reinterpret_cast<R*>(smem + a.lds_state);
-  R* s_c = reinterpret_cast<R*>(smem + a.lds_c);
-  const uint32_t* g_edge = a.edges + cd_e_off;
-  const R* g_w = reinterpret_cast<const R*>(a.weights) + cd_e_off;
-  const int32_t* g_rp = a.row_ptr + cd_rp_off;
-  EdgeRec* l_rec = reinterpret_cast<EdgeRec*>(smem + a.lds_edge);
-  int32_t* l_rp = reinterpret_cast<int32_t*>(smem + a.lds_rp);
-  if (a.csr_in_lds) {
-    for (int k = tid; k < cd_nnz + TVBK_ROW_PAD; k += nthr) {
-      const uint32_t ed = g_edge[k];
-      EdgeRec r;
-      r.coff = (ed & 0xffffu) * cstride;
-      r.delay = ed >> 16;
-      if constexpr (sizeof(R) == 4) {
-        r.w0 = __float_as_uint((float)g_w[k]);
-        r.w1 = 0;
-      } else {
-        const double wd = (double)g_w[k];
-        r.w0 = (uint32_t)__double2loint(wd);
-        r.w1 = (uint32_t)__double2hiint(wd);
-      }
-      l_rec[k] = r;
-    }
-    for (int k = tid; k <= N; k += nthr) l_rp[k] = g_rp[k];
-  }
-  copy_bytes<R>(img, a.img + td_img_off, a.img_bytes);
-  __syncthreads();
-
-  // ---- per-lane instance parameters (phase B lane and phase A lane agree:
-  //      blockDim and 64 are multiples of T) -------------------------------
-  const int t = tid % T;
-  const bool live = t < td_n_inst;
-  const int inst = td_inst0 + (live ? t : 0);
-  const int64_t spatial_row = (int64_t)N * a.prow;   // per-node params: one prow row per node
-  const R* prow = reinterpret_cast<const R*>(a.params) + (int64_t)inst * (a.spatial ? spatial_row : a.prow);
-  R cp[NCP > 0 ? NCP : 1];
-#pragma unroll
-  for (int k = 0; k < NCP; ++k) cp[k] = prow[NDP + k];
-  R gf[NSV];
-#pragma unroll
-  for (int k = 0; k < NSV; ++k) gf[k] = (a.noise != NOISE_NONE) ? prow[NDP + NCP + k] : R(0);
-  R pm[NDP > 0 ? NDP : 1];
-#pragma unroll
-  for (int k = 0; k < NDP; ++k) pm[k] = prow[k];
-  const R dt = (R)a.dt;
-  const uint64_t stream = a.streams ? a.streams[inst] : (uint64_t)inst;
-  R* ring = reinterpret_cast<R*>(a.ring) + td_ring_off;
-
-  // ---- step counters (wave-uniform) --------------------------------------
-  int64_t s = td_step0;
-  int smod = (int)((s - 1) % H);          // (s-1) % H ; s >= 1
-  int bad = 0;
-  const int P = N * NCH;
-  const int NG = 64 / T;        // lane groups per wave
-  const int g = lane / T;
-
-  // ---- phase A: per (node, channel) pair, a group of T lanes reduces the
-  // node's CSR row; the history is gathered with buffer loads at byte offset
-  // slot*sstride + col*cstride + lane_off.  Instantiated once for the LDS and
-  // once for the global CSR so every access has a known address space.
-  constexpr int RMAX = sizeof(R) == 4 ? 32 : 16;
-  const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(ring, (uint32_t)H * sstride);
-  const int rmax = cd_max_row < RMAX ? cd_max_row : RMAX;
-  auto slot_off = [&](uint32_t delay) -> uint32_t {
-    int sl = smod - (int)delay;
-    sl += (sl < 0) ? H : 0;
-    return __umul24((uint32_t)sl, sstride);
-  };
-  auto phase_a = [&](auto getrec, const int32_t* RP, R* cbuf) {
-    for (int q = wave * NG + g; q - g < P; q += nwave * NG) {
-      if (q < P) {
-        const int i = q / NCH, ch = q - (q / NCH) * NCH;
-        const int e0 = RP[i];
-        const int n = RP[i + 1] - e0;
-        const uint32_t lane_off = (uint32_t)(ch * T + t) * (uint32_t)sizeof(R);
-        R acc;
-        if (n <= RMAX) {
-          // three passes: offsets, all loads back to back, then the weights
-          // -- no load result is used before the row's last load issues
-          // (float32 keeps the weights in registers; float64 re-reads them)
-          constexpr bool KEEPW = sizeof(R) == 4;
-          uint32_t off[RMAX];
-          R wv[KEEPW ? RMAX : 1];
-#pragma unroll
-          for (int e = 0; e < RMAX; ++e) {
-            if (e < rmax) {
-              const EdgeRec r = getrec(e0 + e);
-              off[e] = (e < n) ? slot_off(r.delay) + r.coff + lane_off : 0xFFFFFFF0u;
-              if constexpr (KEEPW) wv[e] = rec_weight<R>(r);
-            } else {
-              off[e] = 0xFFFFFFF0u;
-              if constexpr (KEEPW) wv[e] = R(0);
-            }
-          }
-          R v[RMAX];
-#pragma unroll
-          for (int e = 0; e < RMAX; ++e) v[e] = (e < rmax) ? buf_load<R>(rsrc, off[e]) : R(0);
-#pragma unroll
-          for (int e = 0; e < RMAX; ++e) {
-            if constexpr (KEEPW) v[e] = wv[e] * v[e];
-            else v[e] = (e < rmax ? rec_weight<R>(getrec(e0 + e)) : R(0)) * v[e];
-          }
-          acc = reduceat_regs<RMAX, R>(n, v);
-        } else {
-          // long rows: the same order in blocks of eight loads
-          auto ld = [&](int e) -> R {
-            const EdgeRec r = getrec(e0 + e);
-            return buf_load<R>(rsrc, e < n ? slot_off(r.delay) + r.coff + lane_off : 0xFFFFFFF0u);
-          };
-          auto wt = [&](int e) -> R { return rec_weight<R>(getrec(e0 + e)); };
-          acc = reduceat_row_blocked<R>(n, ld, wt);
-        }
-        if (CF == CF_SIGMOID_JR) {
-          cbuf[(i * NCT) * T + t] = C::post(acc, cp);
-        } else {
-          cbuf[(i * NCT + ch) * T + t] = C::post(acc, cp);
-        }
-      }
-    }
-  };
-  auto rec_lds = [&](int e) -> EdgeRec { return l_rec[e]; };
-  auto rec_glb = [&](int e) -> EdgeRec {
-    const uint32_t ed = g_edge[e];
-    EdgeRec r;
-    r.coff = (ed & 0xffffu) * cstride;
-    r.delay = ed >> 16;
-    if constexpr (sizeof(R) == 4) {
-      r.w0 = __float_as_uint((float)g_w[e]);
-      r.w1 = 0;
-    } else {
-      const double wd = (double)g_w[e];
-      r.w0 = (uint32_t)__double2loint(wd);
-      r.w1 = (uint32_t)__double2hiint(wd);
-    }
-    return r;
-  };
-
-  for (int k = 0; k < nsteps; ++k, ++s) {
-    const int buf = k & 1;
-    R* cbuf = s_c + (int64_t)buf * N * NCT * T;
-    // ================= phase A: coupling of step s ========================
-    if (a.csr_in_lds) phase_a(rec_lds, l_rp, cbuf);
-    else phase_a(rec_glb, g_rp, cbuf);
-    __syncthreads();
-    // ================= phase B: integrate, write history, monitors =========
-    const int wslot = (smod + 1 == H) ? 0 : smod + 1;   // s % H
-    for (int i = tid / T; i < N; i += nthr / T) {
-      R c[NCT];
-#pragma unroll
-      for (int v = 0; v < NCT; ++v) c[v] = cbuf[(i * NCT + v) * T + t];
-      R* wr = ring + (wslot * N + i) * rstride + t;
-      update_node<R, M, CF>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s, buf, pm, cp, gf, c, bad,
-                            [&](int ch, R v) { wr[ch * T] = v; });
-    }
-    // bookkeeping of step k+1 into the other buffer (read after the next
-    // phase A barrier; its previous readers finished before this step's)
-    if (tid == 0)
-      for (int m = 0; m < a.n_mon; ++m) step_info(kc.mon[m], k + 1, kc.si[buf ^ 1][m]);
-    smod = wslot;
-    if (a.zero_delay) __syncthreads();
-  }
-  __syncthreads();
-  copy_bytes<R>(a.img + td_img_off, img, a.img_bytes);
-  if (bad && live) atomicOr(a.flags + inst, 1);
-}
-
-// Zero-delay coupling (all idelays 0, horizon 1) as a dense contraction on
-// the matrix cores: per history channel ch, C = W X with W the N x N weights
-// (float32, as the reference stores them) and X the channel values of the
-// tile's 32 instances after the previous step.  v_mfma_f32_16x16x4f32:
-// A (16 x 4): lane l holds A[l % 16][l / 16]; B (4 x 16): B[l / 16][l % 16];
-// D (16 x 16): lane l holds D[4 (l / 16) + v][l % 16], v = 0..3.
-// Wave w owns target rows 16 (w / 2) .. +15 and instances 16 (w % 2) .. +15
-// of the tile; its W fragments stay in registers for the whole call.  X is
-// double-buffered in LDS by step parity: one barrier per step.
-typedef float f4_t __attribute__((ext_vector_type(4)));
-#define TVBK_DENSE_KBMAX 32
-
-template <class M, int CF>
-__global__ void __launch_bounds__(1024) k_dense(StepArgs a) {
-  using R = float;
-  using C = Cfun<M, CF>;
-  constexpr int NSV = M::NSV, NCH = C::NCH, NDP = M::NDP, NCP = C::NCP;
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ KCfg kc;
-
-  if (a.tiles[blockIdx.x].n_steps <= 0) return;
-  const TileDesc* tdg = a.tiles + blockIdx.x;
-  const int conn = tdg->conn;
-  const ConnDesc* cdg = a.conns + conn;
-  const int N = cdg->n_node;
-  const int td_inst0 = tdg->inst0, td_n_inst = tdg->n_inst;
-  const int64_t td_step0 = tdg->step0, td_n_steps = tdg->n_steps, td_ring_off = tdg->ring_off,
-                td_img_off = tdg->img_off;
-  constexpr int T = 32;
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int nsteps = (int)td_n_steps;
-  const int KB = a.dense_kb, KP = 4 * a.dense_kb;
-  const int rb = wave >> 1, cb = wave & 1;
-  const int t = 16 * cb + (lane & 15);           // instance lane of the tile
-  const int r0 = 16 * rb + 4 * (lane >> 4);      // first of the lane's 4 target rows
-
-  if (tid == 0) {
-    kc.td = *tdg;
-    for (int m = 0; m < TVBK_KMAX_MON; ++m) {
-      MonArgs mo = a.mon[m];
-      if (m < a.n_mon) {
-        mo.r0 = (int)(td_step0 % mo.istep);
-        mo.qe0 = td_step0 / mo.istep;
-        mo.rb0 = (int)(td_step0 % mo.ii);
-        mo.qb0 = td_step0 / mo.ii;
-        step_info(mo, 0, kc.si[0][m]);
-      }
-      kc.mon[m] = mo;
-    }
-    for (int o = 0; o < TVBK_KMAX_OBS; ++o) {
-      kc.obs_sv[o] = a.obs_sv[o];
-      for (int u = 0; u < 8; ++u) kc.obs_coef[o][u] = a.obs_coef[o][u];
-    }
-  }
-  unsigned char* img = smem;
-  R* s_x = reinterpret_cast<R*>(smem + a.lds_state);
-  R* X = reinterpret_cast<R*>(smem + a.lds_x);   // [2][NCH][KP][T]
-  R* ring = reinterpret_cast<R*>(a.ring) + td_ring_off;   // one slot: [node][ch][T]
-  copy_bytes<R>(img, a.img + td_img_off, a.img_bytes);
-  for (int q = tid; q < 2 * NCH * KP * T; q += nthr) {
-    const int tt = q % T, kk = (q / T) % KP, ch = (q / (T * KP)) % NCH, b = q / (T * KP * NCH);
-    X[q] = (b == 0 && kk < N) ? ring[(kk * NCH + ch) * T + tt] : R(0);
-  }
-  // W fragments of this wave's 16 rows
-  float afr[TVBK_DENSE_KBMAX];
-  const int arow = 16 * rb + (lane & 15);
-#pragma unroll
-  for (int kb = 0; kb < TVBK_DENSE_KBMAX; ++kb)
-    afr[kb] = (kb < KB) ? a.dense_w[(int64_t)arow * KP + 4 * kb + (lane >> 4)] : 0.0f;
-  __syncthreads();
-
-  const bool live = t < td_n_inst;
-  const int inst = td_inst0 + (live ? t : 0);
-  const R* prow = reinterpret_cast<const R*>(a.params) + (int64_t)inst * a.prow;
-  R cp[NCP > 0 ? NCP : 1];
-#pragma unroll
-  for (int k = 0; k < NCP; ++k) cp[k] = prow[NDP + k];
-  R gf[NSV];
-#pragma unroll
-  for (int k = 0; k < NSV; ++k) gf[k] = (a.noise != NOISE_NONE) ? prow[NDP + NCP + k] : R(0);
-  R pm[NDP > 0 ? NDP : 1];
-#pragma unroll
-  for (int k = 0; k < NDP; ++k) pm[k] = prow[k];
-  const uint64_t stream = a.streams ? a.streams[inst] : (uint64_t)inst;
-  int bad = 0;
-  int64_t s = td_step0;
-  const int krow = lane >> 4;                    // B operand row within a k block
-
-  for (int k = 0; k < nsteps; ++k, ++s) {
-    const int buf = k & 1;
-    const R* Xc = X + (int64_t)buf * NCH * KP * T;
-    R* Xn = X + (int64_t)(buf ^ 1) * NCH * KP * T;
-    f4_t d[NCH];
-#pragma unroll
-    for (int ch = 0; ch < NCH; ++ch) {
-      f4_t acc = {0.f, 0.f, 0.f, 0.f};
-      const R* xb = Xc + (int64_t)ch * KP * T + krow * T + t;
-#pragma unroll
-      for (int kb = 0; kb < TVBK_DENSE_KBMAX; ++kb)
-        if (kb < KB) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[kb], xb[4 * kb * T], acc, 0, 0, 0);
-      d[ch] = acc;
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = r0 + v;
-      if (i < N) {
-        R c[NCH];
-#pragma unroll
-        for (int ch = 0; ch < NCH; ++ch) c[ch] = C::post(d[ch][v], cp);
-        update_node<R, M, CF>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s, buf, pm, cp, gf, c, bad,
-                              [&](int ch, R val) { Xn[((int64_t)ch * KP + i) * T + t] = val; });
-      }
-    }
-    if (tid == 0)
-      for (int m = 0; m < a.n_mon; ++m) step_info(kc.mon[m], k + 1, kc.si[buf ^ 1][m]);
-    __syncthreads();
-  }
-  // the final channel values become the history's single slot
-  const R* Xf = X + (int64_t)(nsteps & 1) * NCH * KP * T;
-  for (int q = tid; q < N * NCH * T; q += nthr) {
-    const int tt = q % T, ch = (q / T) % NCH, i = q / (T * NCH);
-    ring[(i * NCH + ch) * T + tt] = Xf[((int64_t)ch * KP + i) * T + tt];
-  }
-  copy_bytes<R>(a.img + td_img_off, img, a.img_bytes);
-  if (bad && live) atomicOr(a.flags + inst, 1);
-}
-
-// Fills the ring and state of every tile from Philox uniforms in [lo, hi),
-// then bounds them (history.py:150-152).  The state is the value of slot
-// current_step % H = 0.
-template <class R, class M, int CF>
-__global__ void k_init_uniform(InitArgs a) {
-  using C = Cfun<M, CF>;
-  constexpr int NSV = M::NSV, NCH = C::NCH, NDP = M::NDP;
-  const TileDesc td = a.tiles[blockIdx.y];
-  const ConnDesc cd = a.conns[td.conn];
-  const int T = a.T, N = cd.n_node, H = cd.horizon;
-  const int64_t per_tile = (int64_t)H * N * T;
-  R* ring = reinterpret_cast<R*>(a.ring) + td.ring_off;
-  R* st = reinterpret_cast<R*>(a.img + td.img_off + a.lds_state);
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < per_tile; q += (int64_t)gridDim.x * blockDim.x) {
-    const int t = (int)(q % T);
-    const int i = (int)((q / T) % N);
-    const int slot = (int)(q / ((int64_t)T * N));
-    const int inst = td.inst0 + (t < td.n_inst ? t : 0);
-    const uint64_t stream = a.streams ? a.streams[inst] : (uint64_t)inst;
-    R x[NSV];
-    for (int call = 0; call * 4 < NSV; ++call) {
-      U4 c{(uint32_t)i | ((uint32_t)call << 20), (uint32_t)stream, (uint32_t)slot, 0x80000000u};
-      U4 w = philox10(c, a.key0, a.key1);
-      const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
-      for (int u = 0; u < 4 && call * 4 + u < NSV; ++u) {
-        const int v = call * 4 + u;
-        const double uu = (double)ws[u] * 2.3283064365386963e-10;
-        x[v] = (R)(a.lo[v] + (a.hi[v] - a.lo[v]) * uu);
-      }
-    }
-    M::template bound<R>(x);
-    const R* prow = reinterpret_cast<const R*>(a.params) + (int64_t)inst * a.prow;
-    R cp[C::NCP > 0 ? C::NCP : 1];
-    for (int k = 0; k < C::NCP; ++k) cp[k] = prow[NDP + k];
-    for (int ch = 0; ch < NCH; ++ch) ring[((int64_t)slot * N + i) * NCH * T + ch * T + t] = C::template channel<R>(x, ch, cp);
-    if (slot == 0)
-      for (int v = 0; v < NSV; ++v) st[(i * NSV + v) * T + t] = x[v];
-  }
-}
-
-template <class R>
-__global__ void k_scatter(bool gather, R* ring, R* st, R* hist, R* state, int H, int N, int nch, int nsv, int T,
+template <class RH, class R>
+__global__ void k_scatter(bool gather, RH* ring, R* st, RH* hist, R* state, int H, int N, int nch, int nsv, int T,
                           int t) {
   const int64_t nh = (int64_t)H * N * nch;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nh + (int64_t)nsv * N;
        q += (int64_t)gridDim.x * blockDim.x) {
     if (q < nh) {
       const int i = (int)(q % N), ch = (int)((q / N) % nch), slot = (int)(q / ((int64_t)N * nch));
-      R* r = ring + (((int64_t)slot * N + i) * nch + ch) * T + t;
+      RH* r = ring + (((int64_t)slot * N + i) * nch + ch) * T + t;
       if (gather) hist[q] = *r; else *r = hist[q];
     } else {
       const int64_t u = q - nh;
@@ -647,28 +47,12 @@ __global__ void k_debug_slots(const uint32_t* edges, int nnz, int H, int64_t ste
 // ---------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------
-template <class R, class M, int CF>
-static hipError_t launch_t(const StepArgs& a, int ntiles, int nthreads, size_t lds, hipStream_t st) {
-  auto kern = k_step<R, M, CF>;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(ntiles), dim3(nthreads), lds, st, a);
-  return hipGetLastError();
-}
-
-template <class R, class M, int CF>
-static hipError_t launch_init_t(const InitArgs& a, int ntiles, hipStream_t st) {
-  hipLaunchKernelGGL((k_init_uniform<R, M, CF>), dim3(256, ntiles), dim3(256), 0, st, a);
-  return hipGetLastError();
-}
-
-template <class R>
+template <class RH, class R>
 static hipError_t dispatch(int model, int cf, bool init, const StepArgs* a, const InitArgs* ia, int ntiles,
                            int nthreads, size_t lds, hipStream_t st) {
 #define TVBK_CASE(MODEL, CFV)                                                       \
-  if (init) return launch_init_t<R, MODEL, CFV>(*ia, ntiles, st);                   \
-  return launch_t<R, MODEL, CFV>(*a, ntiles, nthreads, lds, st);
+  if (init) return launch_init_t<RH, R, MODEL, CFV>(*ia, ntiles, st);               \
+  return launch_step_t<RH, R, MODEL, CFV>(*a, ntiles, nthreads, lds, st);
   if (model == 0 && cf == CF_LINEAR) { TVBK_CASE(G2D, CF_LINEAR) }
   if (model == 1 && cf == CF_LINEAR) { TVBK_CASE(MPR, CF_LINEAR) }
   if (model == 2 && cf == CF_LINEAR) { TVBK_CASE(JR, CF_LINEAR) }
@@ -679,37 +63,45 @@ static hipError_t dispatch(int model, int cf, bool init, const StepArgs* a, cons
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_dense(int model, const StepArgs& a, int ntiles, int nthreads, size_t lds, hipStream_t st) {
-  const void* fn = nullptr;
-  if (model == 0) fn = reinterpret_cast<const void*>(k_dense<G2D, CF_LINEAR>);
-  else if (model == 1) fn = reinterpret_cast<const void*>(k_dense<MPR, CF_LINEAR>);
-  else return hipErrorInvalidValue;
-  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
-  if (model == 0) hipLaunchKernelGGL((k_dense<G2D, CF_LINEAR>), dim3(ntiles), dim3(nthreads), lds, st, a);
-  else hipLaunchKernelGGL((k_dense<MPR, CF_LINEAR>), dim3(ntiles), dim3(nthreads), lds, st, a);
-  return hipGetLastError();
+hipError_t launch_dense(int precision, int model, const StepArgs& a, int ntiles, int nthreads, size_t lds,
+                        hipStream_t st) {
+  if (precision == 2) {
+    if (model == 0) return launch_dense_t<double, G2D>(a, ntiles, nthreads, lds, st);
+    if (model == 1) return launch_dense_t<double, MPR>(a, ntiles, nthreads, lds, st);
+    return hipErrorInvalidValue;
+  }
+  if (precision != 0) return hipErrorInvalidValue;
+  if (model == 0) return launch_dense_t<float, G2D>(a, ntiles, nthreads, lds, st);
+  if (model == 1) return launch_dense_t<float, MPR>(a, ntiles, nthreads, lds, st);
+  return hipErrorInvalidValue;
 }
 
+// precision: 0 float history and state, 1 double both, 2 float history with
+// double state
 hipError_t launch_step(int precision, int model, int cf, const StepArgs& a, int ntiles, int nthreads, size_t lds,
                        hipStream_t st) {
-  if (precision == 1) return dispatch<double>(model, cf, false, &a, nullptr, ntiles, nthreads, lds, st);
-  return dispatch<float>(model, cf, false, &a, nullptr, ntiles, nthreads, lds, st);
+  if (precision == 1) return dispatch<double, double>(model, cf, false, &a, nullptr, ntiles, nthreads, lds, st);
+  if (precision == 2) return dispatch<float, double>(model, cf, false, &a, nullptr, ntiles, nthreads, lds, st);
+  return dispatch<float, float>(model, cf, false, &a, nullptr, ntiles, nthreads, lds, st);
 }
 
 hipError_t launch_init(int precision, int model, int cf, const InitArgs& a, int ntiles, hipStream_t st) {
-  if (precision == 1) return dispatch<double>(model, cf, true, nullptr, &a, ntiles, 0, 0, st);
-  return dispatch<float>(model, cf, true, nullptr, &a, ntiles, 0, 0, st);
+  if (precision == 1) return dispatch<double, double>(model, cf, true, nullptr, &a, ntiles, 0, 0, st);
+  if (precision == 2) return dispatch<float, double>(model, cf, true, nullptr, &a, ntiles, 0, 0, st);
+  return dispatch<float, float>(model, cf, true, nullptr, &a, ntiles, 0, 0, st);
 }
 
 hipError_t launch_scatter(int precision, bool gather, void* ring, unsigned char* img_state, void* hist,
                           void* state, int H, int N, int nch, int nsv, int T, int t, hipStream_t st) {
   if (precision == 1)
-    hipLaunchKernelGGL(k_scatter<double>, dim3(64), dim3(256), 0, st, gather, (double*)ring, (double*)img_state,
-                       (double*)hist, (double*)state, H, N, nch, nsv, T, t);
+    hipLaunchKernelGGL((k_scatter<double, double>), dim3(64), dim3(256), 0, st, gather, (double*)ring,
+                       (double*)img_state, (double*)hist, (double*)state, H, N, nch, nsv, T, t);
+  else if (precision == 2)
+    hipLaunchKernelGGL((k_scatter<float, double>), dim3(64), dim3(256), 0, st, gather, (float*)ring,
+                       (double*)img_state, (float*)hist, (double*)state, H, N, nch, nsv, T, t);
   else
-    hipLaunchKernelGGL(k_scatter<float>, dim3(64), dim3(256), 0, st, gather, (float*)ring, (float*)img_state,
-                       (float*)hist, (float*)state, H, N, nch, nsv, T, t);
+    hipLaunchKernelGGL((k_scatter<float, float>), dim3(64), dim3(256), 0, st, gather, (float*)ring,
+                       (float*)img_state, (float*)hist, (float*)state, H, N, nch, nsv, T, t);
   return hipGetLastError();
 }
 

@@ -82,10 +82,11 @@ struct InitArgs {
 hipError_t launch_step(int precision, int model, int cf, const StepArgs& a, int ntiles, int nthreads, size_t lds,
                        hipStream_t st);
 hipError_t launch_init(int precision, int model, int cf, const InitArgs& a, int ntiles, hipStream_t st);
-hipError_t launch_dense(int model, const StepArgs& a, int ntiles, int nthreads, size_t lds, hipStream_t st);
+hipError_t launch_dense(int precision, int model, const StepArgs& a, int ntiles, int nthreads, size_t lds,
+                        hipStream_t st);
 hipError_t launch_debug_slots(const uint32_t* edges, int nnz, int H, int64_t step, int32_t* out, hipStream_t st);
-// scatter a compact history [H][nch][N] / state [nsv][N] (plan precision)
-// into lane t of a tile, and the reverse gather.
+// scatter a compact history [H][nch][N] (history precision) / state
+// [nsv][N] (state precision) into lane t of a tile, and the reverse gather.
 hipError_t launch_scatter(int precision, bool gather, void* ring, unsigned char* img_state, void* hist,
                           void* state, int H, int N, int nch, int nsv, int T, int t, hipStream_t st);
 

@@ -21,7 +21,7 @@ OK, EINVAL, ENOMEM, EHIP, EUNSUPPORTED, ESTATE = 0, -1, -2, -3, -4, -5
 MODEL_G2D, MODEL_MPR, MODEL_JR, MODEL_SUM = 0, 1, 2, 3
 CFUN_LINEAR, CFUN_SIGMOID_JR, CFUN_IDENTITY = 0, 1, 2
 EULER_DET, EULER_STO, HEUN_DET, HEUN_STO, IDENTITY = 0, 1, 2, 3, 4
-F32, F64 = 0, 1
+F32, F64, MIXED = 0, 1, 2   # MIXED: float32 history, weights and sums; float64 state
 NOISE_NONE, NOISE_INJECTED, NOISE_PHILOX = 0, 1, 2
 MON_RAW, MON_TAVG, MON_BOLD = 0, 1, 2
 KERNEL_AUTO, KERNEL_BATCH_LANE, KERNEL_EDGE_LANE, KERNEL_DENSE_MFMA = 0, 1, 2, 3

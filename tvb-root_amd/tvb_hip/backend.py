@@ -29,7 +29,16 @@ CFUNS = {"Linear": (L.CFUN_LINEAR, ("a", "b")),
 SCHEMES = {"HeunDeterministic": L.HEUN_DET, "HeunStochastic": L.HEUN_STO,
            "EulerDeterministic": L.EULER_DET, "EulerStochastic": L.EULER_STO}
 MONITORS = {"Raw": L.MON_RAW, "TemporalAverage": L.MON_TAVG, "Bold": L.MON_BOLD}
-PRECISIONS = {"fp32": L.F32, "float32": L.F32, "fp64": L.F64, "float64": L.F64}
+PRECISIONS = {"fp32": L.F32, "float32": L.F32, "fp64": L.F64, "float64": L.F64, "mixed": L.MIXED}
+
+
+def precision_id(precision, cfun_id):
+    """Plan precision for a name.  "auto" is the reference's own mix (float32
+    history and sums, float64 state: "mixed") where the device offers it,
+    float32 otherwise (SigmoidalJansenRit)."""
+    if precision == "auto":
+        return L.F32 if cfun_id == L.CFUN_SIGMOID_JR else L.MIXED
+    return PRECISIONS[precision]
 
 
 def _name(obj):
@@ -162,13 +171,13 @@ class HipBackend:
         state = np.asarray(sim.current_state, np.float64)[..., 0]
         return np.ascontiguousarray(cv), state
 
-    def build_plan(self, sim, precision="fp32", noise="legacy", max_steps=65536, kernel="auto",
+    def build_plan(self, sim, precision="auto", noise="legacy", max_steps=65536, kernel="auto",
                    tile=0, waves=0, seed=None, stream=None):
         """A one-instance Plan from a configured simulator, with its history
         and state uploaded."""
         self.check_compatibility(sim)
-        prec = PRECISIONS[precision]
         model_id, cfun_id, cfun_params, scheme, stochastic, specs = self._components(sim)
+        prec = precision_id(precision, cfun_id)
         N = sim.connectivity.number_of_regions
         csr = CSR(sim.connectivity.weights, sim.connectivity.idelays, sim.connectivity.horizon)
         params, spatial = _param_vector(sim, model_id, cfun_params, stochastic, N)
@@ -197,7 +206,7 @@ class HipBackend:
         return plan
 
     # --------------------------------------------------------------- running
-    def run_sim(self, sim, nstep=None, simulation_length=None, precision="fp32", noise="legacy",
+    def run_sim(self, sim, nstep=None, simulation_length=None, precision="auto", noise="legacy",
                 chunk=65536, kernel="auto", write_back=True, **_ignored):
         """Run a configured simulator for nstep steps (or simulation_length ms).
 
@@ -266,8 +275,8 @@ class HipBackend:
             raise ValueError("no simulators")
         for s in sims:
             self.check_compatibility(s)
-        prec = PRECISIONS[precision]
         model_id, cfun_id, cfun_params, scheme, stochastic, specs = self._components(sims[0])
+        prec = precision_id(precision, cfun_id)
         dt = sims[0].integrator.dt
         key0 = (model_id, cfun_id, scheme, [(m.kind, m.istep, tuple(m.voi)) for m in specs])
         conns, inst_conn, sig = [], [], []
@@ -344,8 +353,8 @@ class HipBackend:
         init="uniform" draws each instance's history on the device uniformly
         in the model's state_variable_range (seeded by init_seed)."""
         self.check_compatibility(sim)
-        prec = PRECISIONS[precision]
         model_id, cfun_id, cfun_params, scheme, stochastic, specs = self._components(sim)
+        prec = precision_id(precision, cfun_id)
         N = sim.connectivity.number_of_regions
         base, spatial = _param_vector(sim, model_id, cfun_params, stochastic, N)
         if spatial:

@@ -113,7 +113,8 @@ class Plan:
         d.max_steps = self.max_steps
         d.tile, d.waves = tile, waves
         self._desc = d
-        self.esize = 8 if precision == L.F64 else 4
+        self.esize = 8 if precision in (L.F64, L.MIXED) else 4   # state and Raw/TAvg outputs
+        self.hsize = 8 if precision == L.F64 else 4                # history ring
         self._z_keep = None
         if _dry:
             nb = C.c_int64()

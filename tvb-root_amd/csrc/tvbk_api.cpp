@@ -130,10 +130,11 @@ struct tvbk_plan {
   int lds_bold = 0, lds_state = 0, lds_c = 0, lds_edge = 0, lds_w = 0, lds_rp = 0, lds_total = 0;
   int64_t img_bytes = 0;
   bool csr_in_lds = false;
+  bool uniform_rows = false;   // batch-lane, n_channel * T a multiple of 32
   int mon_lds_off[TVBK_MAX_MON] = {0};
   int mon_Q[TVBK_MAX_MON] = {0};
   // device buffers
-  DevBuf d_tiles, d_conns, d_edges, d_w, d_rp, d_ring, d_img, d_bacc, d_params, d_flags, d_streams, d_z;
+  DevBuf d_tiles, d_conns, d_edges, d_w, d_rp, d_ring, d_img, d_bacc, d_params, d_flags, d_streams, d_z, d_srec;
   DevBuf d_taps[TVBK_MAX_MON], d_out[TVBK_MAX_MON];
   DevBuf d_dense;             // dense zero-delay kernel: W [16*RB][KP] float
   int dense_kb = 0, dense_rb = 0, lds_x = 0;
@@ -217,8 +218,8 @@ const char* tvbk_last_error(tvbk_plan* plan) { return plan ? plan->err.c_str() :
 
 static void destroy(tvbk_plan* p) {
   (void)hipStreamSynchronize(p->stream);   // no pending copy or kernel touches the buffers
-  DevBuf* bufs[] = {&p->d_tiles, &p->d_conns, &p->d_edges, &p->d_w,      &p->d_rp,     &p->d_ring,
-                    &p->d_img,   &p->d_bacc,  &p->d_params, &p->d_flags, &p->d_streams, &p->d_z};
+  DevBuf* bufs[] = {&p->d_tiles, &p->d_conns,  &p->d_edges, &p->d_w,       &p->d_rp, &p->d_ring, &p->d_img,
+                    &p->d_bacc,  &p->d_params, &p->d_flags, &p->d_streams, &p->d_z,  &p->d_srec};
   for (DevBuf* b : bufs) dev_free(b);
   for (int m = 0; m < TVBK_MAX_MON; ++m) {
     dev_free(&p->d_taps[m]);
@@ -453,11 +454,16 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
     if (layout_lds(p, 1, max_lds, true) != TVBK_OK) { p->err = "node state does not fit in LDS"; return bail(TVBK_EUNSUPPORTED); }
   }
   p->T = T;
+  // wave-uniform rows: one target node per wave, edge records through the
+  // scalar cache (no CSR in LDS)
+  p->uniform_rows = kernel == TVBK_KERNEL_BATCH_LANE && (p->nch * T) % 32 == 0;
+  if (p->uniform_rows) layout_lds(p, T, max_lds, false);
   {
     const int P = p->max_n * std::max(p->nch, 1);
     int nw;
     if (kernel == TVBK_KERNEL_DENSE_MFMA) nw = 2 * p->dense_rb;
     else if (d.waves) nw = d.waves;
+    else if (p->uniform_rows) nw = std::min(16, p->max_n);
     else if (T > 1) nw = std::min(16, std::max(1, (P + 64 / T - 1) / (64 / T)));
     else nw = std::min(16, std::max(1, (std::max(P, p->max_n) + 63) / 64));
     const int max_waves = p->hsize == 8 ? 8 : 16;   // the float64-history kernels are built for 512 threads
@@ -546,7 +552,8 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
     size_t tot = p->conns.size() * sizeof(ConnDesc) + edges.size() * 4 + 16 + rp.size() * 4 +
                  (w64.size() * p->hsize + 16) + prm.size() * p->esize + (d.inst_stream ? (size_t)B * 8 : 0) +
                  (size_t)ring_off * p->hsize + (size_t)img_off + (size_t)bacc_off * 8 + (size_t)B * 4 +
-                 p->tiles.size() * sizeof(TileDesc);
+                 p->tiles.size() * sizeof(TileDesc) +
+                 (p->uniform_rows ? (size_t)(p->conns[0].nnz + TVBK_ROW_PAD) * 16 : 0);
     for (int m = 0; m < d.n_mon; ++m) {
       if (d.mon[m].kind == TVBK_MON_BOLD) tot += p->taps_host[m].size() * 8;
       const int64_t cap = d.max_steps / d.mon[m].istep + 1;
@@ -596,6 +603,32 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   }
   if (d.inst_stream) {
     if ((rc = up(&p->d_streams, d.inst_stream, (size_t)B * 8))) return bail(rc);
+  }
+  std::vector<uint32_t> srec;   // host source of an async upload: alive until the final sync
+  if (p->uniform_rows) {
+    // {delay * slot bytes, source * node bytes, weight bits}: one record per
+    // CSR entry of the shared connectome, padding entries included
+    const ConnDesc& cd = p->conns[0];
+    const uint32_t cstride = (uint32_t)(p->nch * T * p->hsize), sstride = (uint32_t)cd.n_node * cstride;
+    const int64_t ne = cd.nnz + TVBK_ROW_PAD;
+    srec.assign((size_t)ne * 4, 0u);
+    for (int64_t e = 0; e < ne; ++e) {
+      const uint32_t ed = edges[cd.e_off + e];
+      uint32_t* r = &srec[(size_t)e * 4];
+      r[0] = (ed >> 16) * sstride;
+      r[1] = (ed & 0xffffu) * cstride;
+      if (p->hsize == 8) {
+        uint64_t bits;
+        std::memcpy(&bits, &w64[cd.e_off + e], 8);
+        r[2] = (uint32_t)bits;
+        r[3] = (uint32_t)(bits >> 32);
+      } else {
+        const float wf = (float)w64[cd.e_off + e];
+        std::memcpy(&r[2], &wf, 4);
+        r[3] = 0;
+      }
+    }
+    if ((rc = up(&p->d_srec, srec.data(), srec.size() * 4))) return bail(rc);
   }
   if (kernel == TVBK_KERNEL_DENSE_MFMA) {
     const tvbk_connectome& cn = d.conns[0];
@@ -913,6 +946,8 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
   a.scheme = d.scheme;
   a.noise = d.noise;
   a.csr_in_lds = p->csr_in_lds ? 1 : 0;
+  a.srec = (const uint4*)p->d_srec.p;
+  a.uniform_rows = p->uniform_rows ? 1 : 0;
   int zero_delay = 0;
   for (const ConnDesc& cd : p->conns) zero_delay |= (cd.d_min == 0 && cd.nnz > 0);
   a.zero_delay = zero_delay;

@@ -45,6 +45,9 @@ struct StepArgs {
   const uint32_t* edges;
   const void* weights;
   const int32_t* row_ptr;
+  // per-plan edge records for the wave-uniform row path: {delay * slot
+  // bytes, source * node bytes, weight bits lo, hi} (one shared connectome)
+  const uint4* srec;
   void* ring;
   unsigned char* img;
   double* bacc;
@@ -54,6 +57,7 @@ struct StepArgs {
   int32_t* flags;
   const uint64_t* streams;
   int32_t prow, spatial, T, n_mon, scheme, noise, csr_in_lds, zero_delay;
+  int32_t uniform_rows;   // one target node per wave, edge records in scalar registers
   uint32_t key0, key1;
   double dt, sqrt_dt;
   int32_t lds_bold, lds_state, lds_c, lds_edge, lds_w, lds_rp;

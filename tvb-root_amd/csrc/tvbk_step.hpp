@@ -71,6 +71,27 @@ template <> __device__ inline double rec_weight<double>(const EdgeRec& e) {
   return __hiloint2double((int)e.w1, (int)e.w0);
 }
 
+// Edge records of the wave-uniform row path, read through the scalar cache:
+// {delay * slot bytes, source node * node bytes, weight bits lo, hi}.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) u32x4 ScalarRec;
+typedef const __attribute__((address_space(4))) int32_t ScalarInt;
+
+template <class R> __device__ inline R srec_weight(u32x4 r);
+template <> __device__ inline float srec_weight<float>(u32x4 r) { return __uint_as_float(r.z); }
+template <> __device__ inline double srec_weight<double>(u32x4 r) {
+  return __hiloint2double((int)r.w, (int)r.z);
+}
+
+// history value at vector offset voff (per lane) + scalar offset soff
+template <class R> __device__ inline R buf_load2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <> __device__ inline float buf_load2<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <> __device__ inline double buf_load2<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
 // Per-step monitor bookkeeping for step s = s0 + k (uniform): emission when
 // s % istep == 0 (sample index within the call), Bold push when s % ii == 0
 // (interim sample index mp = s / ii) and emission index kk = s / istep.
@@ -408,6 +429,70 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
       }
     }
   };
+  // wave-uniform rows (NCH * T a multiple of 32, one shared connectome): a
+  // wave reduces all channels and instances of one target node; the row's
+  // edge records come through the scalar cache and each edge's history
+  // offset is a scalar buffer offset, so an edge costs each lane one load,
+  // one multiply and one add
+  ScalarRec* c_rec = (ScalarRec*)(a.srec + cd_e_off);
+  ScalarInt* c_rp = (ScalarInt*)(a.row_ptr + cd_rp_off);
+  const uint32_t Hs = (uint32_t)H * sstride;
+  auto phase_a_uni = [&](RH* cbuf) {
+    // slot (s - 1) % H: wave-uniform, kept in scalar registers
+    const uint32_t base = __builtin_amdgcn_readfirstlane((uint32_t)smod * sstride);
+    const uint32_t hs = __builtin_amdgcn_readfirstlane(Hs);
+    const int QN = NCH * T;                             // lanes per target node
+    // (slot byte offsets stay below 2^31: a tile's history is < 2 GiB)
+    auto soff = [&](u32x4 r) -> uint32_t {
+      int o = (int)base - (int)r.x;
+      o += (o < 0) ? (int)hs : 0;
+      return (uint32_t)o + r.y;
+    };
+    for (int j = 0; j * 64 < QN; ++j) {                 // uniform trip count
+      const int q = j * 64 + lane;
+      // lanes past the node's values read lane 0's (discarded)
+      const uint32_t lane_off = q < QN ? (uint32_t)q * (uint32_t)sizeof(RH) : 0u;
+      for (int i = __builtin_amdgcn_readfirstlane(wave); i < N; i += nwave) {   // i uniform: scalar loads
+        const int e0 = __builtin_amdgcn_readfirstlane(c_rp[i]);
+        const int n = __builtin_amdgcn_readfirstlane(c_rp[i + 1]) - e0;
+        RH acc;
+        if (n <= RMAX) {
+          // loads in blocks of four under one uniform branch; entries of the
+          // last block past the row's end re-read its first entry (a cache
+          // hit) and are never summed
+          RH v[RMAX];
+          const uint32_t so0 = soff(c_rec[e0]);
+#pragma unroll
+          for (int b = 0; b < RMAX / 4; ++b) {
+            if (4 * b < n) {
+#pragma unroll
+              for (int k = 0; k < 4; ++k) {
+                const int e = 4 * b + k;
+                const uint32_t so = e < n ? soff(c_rec[e0 + e]) : so0;
+                v[e] = buf_load2<RH>(rsrc, lane_off, __builtin_amdgcn_readfirstlane(so));
+              }
+            } else {
+#pragma unroll
+              for (int k = 0; k < 4; ++k) v[4 * b + k] = RH(0);
+            }
+          }
+#pragma unroll
+          for (int e = 0; e < RMAX; ++e) v[e] = srec_weight<RH>(c_rec[e0 + e]) * v[e];
+          acc = reduceat_regs<RMAX, RH>(n, v);
+        } else {
+          auto ld = [&](int e) -> RH {
+            return e < n ? buf_load2<RH>(rsrc, lane_off, __builtin_amdgcn_readfirstlane(soff(c_rec[e0 + e]))) : RH(0);
+          };
+          auto wt = [&](int e) -> RH { return srec_weight<RH>(c_rec[e0 + e]); };
+          acc = reduceat_row_blocked<RH>(n, ld, wt);
+        }
+        if (q < QN) {
+          const int ch = q / T, tq = q - (q / T) * T;
+          cbuf[(i * NCT + (CF == CF_SIGMOID_JR ? 0 : ch)) * T + tq] = acc;
+        }
+      }
+    }
+  };
   auto rec_lds = [&](int e) -> EdgeRec { return l_rec[e]; };
   auto rec_glb = [&](int e) -> EdgeRec {
     const uint32_t ed = g_edge[e];
@@ -429,7 +514,8 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     const int buf = k & 1;
     RH* cbuf = s_c + (int64_t)buf * N * NCT * T;
     // ================= phase A: coupling of step s ========================
-    if (a.csr_in_lds) phase_a(rec_lds, l_rp, cbuf);
+    if (a.uniform_rows) phase_a_uni(cbuf);
+    else if (a.csr_in_lds) phase_a(rec_lds, l_rp, cbuf);
     else phase_a(rec_glb, g_rp, cbuf);
     __syncthreads();
     // ================= phase B: integrate, write history, monitors =========

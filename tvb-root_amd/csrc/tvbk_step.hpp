@@ -437,59 +437,71 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   ScalarRec* c_rec = (ScalarRec*)(a.srec + cd_e_off);
   ScalarInt* c_rp = (ScalarInt*)(a.row_ptr + cd_rp_off);
   const uint32_t Hs = (uint32_t)H * sstride;
-  auto phase_a_uni = [&](RH* cbuf) {
-    // slot (s - 1) % H: wave-uniform, kept in scalar registers
-    const uint32_t base = __builtin_amdgcn_readfirstlane((uint32_t)smod * sstride);
-    const uint32_t hs = __builtin_amdgcn_readfirstlane(Hs);
-    const int QN = NCH * T;                             // lanes per target node
-    // (slot byte offsets stay below 2^31: a tile's history is < 2 GiB)
-    auto soff = [&](u32x4 r) -> uint32_t {
-      int o = (int)base - (int)r.x;
-      o += (o < 0) ? (int)hs : 0;
-      return (uint32_t)o + r.y;
+  const uint32_t hs = __builtin_amdgcn_readfirstlane(Hs);
+  const int QN = NCH * T;                               // lanes per target node
+  // history byte offset of an edge record for slot base `base` ((s-1) % H
+  // times the slot bytes); offsets stay below 2^31 (a tile's history < 2 GiB)
+  auto soff = [&](uint32_t base, u32x4 r) -> uint32_t {
+    int o = (int)base - (int)r.x;
+    o += (o < 0) ? (int)hs : 0;
+    return (uint32_t)o + r.y;
+  };
+  // issue the loads of row i (n <= RMAX): blocks of four under one uniform
+  // branch; entries of the last block past the row's end re-read its first
+  // entry (a cache hit) and are never summed.  Returns n; rows longer than
+  // RMAX load nothing here.
+  auto row_issue = [&](int i, uint32_t base, uint32_t lane_off, RH* v, int& e0) -> int {
+    e0 = __builtin_amdgcn_readfirstlane(c_rp[i]);
+    const int n = __builtin_amdgcn_readfirstlane(c_rp[i + 1]) - e0;
+    if (n <= RMAX) {
+      const uint32_t so0 = soff(base, c_rec[e0]);
+#pragma unroll
+      for (int b = 0; b < RMAX / 4; ++b) {
+        if (4 * b < n) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int e = 4 * b + k;
+            const uint32_t so = e < n ? soff(base, c_rec[e0 + e]) : so0;
+            v[e] = buf_load2<RH>(rsrc, lane_off, __builtin_amdgcn_readfirstlane(so));
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) v[4 * b + k] = RH(0);
+        }
+      }
+    }
+    return n;
+  };
+  // weights and the numpy reduceat order; long rows are loaded here
+  auto row_finish = [&](int e0, int n, uint32_t base, uint32_t lane_off, RH* v) -> RH {
+    if (n <= RMAX) {
+#pragma unroll
+      for (int e = 0; e < RMAX; ++e) v[e] = srec_weight<RH>(c_rec[e0 + e]) * v[e];
+      return reduceat_regs<RMAX, RH>(n, v);
+    }
+    auto ld = [&](int e) -> RH {
+      return e < n ? buf_load2<RH>(rsrc, lane_off, __builtin_amdgcn_readfirstlane(soff(base, c_rec[e0 + e]))) : RH(0);
     };
+    auto wt = [&](int e) -> RH { return srec_weight<RH>(c_rec[e0 + e]); };
+    return reduceat_row_blocked<RH>(n, ld, wt);
+  };
+  auto row_store = [&](RH* cbuf, int i, int q, RH acc) {
+    if (q < QN) {
+      const int ch = q / T, tq = q - (q / T) * T;
+      cbuf[(i * NCT + (CF == CF_SIGMOID_JR ? 0 : ch)) * T + tq] = acc;
+    }
+  };
+  auto phase_a_uni = [&](RH* cbuf) {
+    const uint32_t base = __builtin_amdgcn_readfirstlane((uint32_t)smod * sstride);   // slot (s - 1) % H
     for (int j = 0; j * 64 < QN; ++j) {                 // uniform trip count
       const int q = j * 64 + lane;
       // lanes past the node's values read lane 0's (discarded)
       const uint32_t lane_off = q < QN ? (uint32_t)q * (uint32_t)sizeof(RH) : 0u;
       for (int i = __builtin_amdgcn_readfirstlane(wave); i < N; i += nwave) {   // i uniform: scalar loads
-        const int e0 = __builtin_amdgcn_readfirstlane(c_rp[i]);
-        const int n = __builtin_amdgcn_readfirstlane(c_rp[i + 1]) - e0;
-        RH acc;
-        if (n <= RMAX) {
-          // loads in blocks of four under one uniform branch; entries of the
-          // last block past the row's end re-read its first entry (a cache
-          // hit) and are never summed
-          RH v[RMAX];
-          const uint32_t so0 = soff(c_rec[e0]);
-#pragma unroll
-          for (int b = 0; b < RMAX / 4; ++b) {
-            if (4 * b < n) {
-#pragma unroll
-              for (int k = 0; k < 4; ++k) {
-                const int e = 4 * b + k;
-                const uint32_t so = e < n ? soff(c_rec[e0 + e]) : so0;
-                v[e] = buf_load2<RH>(rsrc, lane_off, __builtin_amdgcn_readfirstlane(so));
-              }
-            } else {
-#pragma unroll
-              for (int k = 0; k < 4; ++k) v[4 * b + k] = RH(0);
-            }
-          }
-#pragma unroll
-          for (int e = 0; e < RMAX; ++e) v[e] = srec_weight<RH>(c_rec[e0 + e]) * v[e];
-          acc = reduceat_regs<RMAX, RH>(n, v);
-        } else {
-          auto ld = [&](int e) -> RH {
-            return e < n ? buf_load2<RH>(rsrc, lane_off, __builtin_amdgcn_readfirstlane(soff(c_rec[e0 + e]))) : RH(0);
-          };
-          auto wt = [&](int e) -> RH { return srec_weight<RH>(c_rec[e0 + e]); };
-          acc = reduceat_row_blocked<RH>(n, ld, wt);
-        }
-        if (q < QN) {
-          const int ch = q / T, tq = q - (q / T) * T;
-          cbuf[(i * NCT + (CF == CF_SIGMOID_JR ? 0 : ch)) * T + tq] = acc;
-        }
+        RH v[RMAX];
+        int e0;
+        const int n = row_issue(i, base, lane_off, v, e0);
+        row_store(cbuf, i, q, row_finish(e0, n, base, lane_off, v));
       }
     }
   };
@@ -510,6 +522,50 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     return r;
   };
 
+  // one (node i, lane t) update of phase B for step s = s0 + k
+  auto b_item = [&](int i, const RH* cbuf, int k, int64_t s_, int buf, int wslot) {
+    R c[NCT];
+#pragma unroll
+    for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(i * NCT + v) * T + t], cp);
+    RH* wr = ring + (wslot * N + i) * rstride + t;
+    update_node<R, M, CF, RH>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s_, buf, pm, cp, gf, c, bad,
+                              [&](int ch, R v) { wr[ch * T] = (RH)v; });
+  };
+
+  // ---- pipelined loop (uniform rows, one wave pass per node, every delay
+  // >= 1): the coupling of step s+1 reads slots <= s-1, written before the
+  // previous barrier, so each wave interleaves its rows of phase A(s+1) with
+  // its updates of phase B(s) -- a row's loads are in flight while the wave
+  // integrates -- and one barrier per step orders both against the next.
+  if (a.uniform_rows && !a.zero_delay && QN <= 64) {
+    phase_a_uni(s_c);                                   // coupling of the first step
+    __syncthreads();
+    const int w0 = __builtin_amdgcn_readfirstlane(wave);
+    const int nrow = (N - w0 + nwave - 1) / nwave;      // rows of this wave
+    const int npass = (N + nthr / T - 1) / (nthr / T);  // phase B passes
+    const int nu = nrow > npass ? nrow : npass;
+    const uint32_t lane_off = lane < QN ? (uint32_t)lane * (uint32_t)sizeof(RH) : 0u;
+    for (int k = 0; k < nsteps; ++k, ++s) {
+      const int buf = k & 1;
+      const RH* cbuf = s_c + (int64_t)buf * N * NCT * T;
+      RH* cnext = s_c + (int64_t)(buf ^ 1) * N * NCT * T;
+      const int wslot = (smod + 1 == H) ? 0 : smod + 1;   // s % H
+      const uint32_t base_n = __builtin_amdgcn_readfirstlane((uint32_t)wslot * sstride);   // (s+1-1) % H
+      for (int u = 0; u < nu; ++u) {
+        const int ra = w0 + u * nwave;                  // uniform
+        const int ib = tid / T + u * (nthr / T);
+        RH v[RMAX];
+        int e0 = 0, n = 0;
+        if (ra < N) n = row_issue(ra, base_n, lane_off, v, e0);
+        if (ib < N) b_item(ib, cbuf, k, s, buf, wslot);
+        if (ra < N) row_store(cnext, ra, lane, row_finish(e0, n, base_n, lane_off, v));
+      }
+      if (tid == 0)
+        for (int m = 0; m < a.n_mon; ++m) step_info(kc.mon[m], k + 1, kc.si[buf ^ 1][m]);
+      smod = wslot;
+      __syncthreads();
+    }
+  } else
   for (int k = 0; k < nsteps; ++k, ++s) {
     const int buf = k & 1;
     RH* cbuf = s_c + (int64_t)buf * N * NCT * T;
@@ -520,14 +576,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     __syncthreads();
     // ================= phase B: integrate, write history, monitors =========
     const int wslot = (smod + 1 == H) ? 0 : smod + 1;   // s % H
-    for (int i = tid / T; i < N; i += nthr / T) {
-      R c[NCT];
-#pragma unroll
-      for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(i * NCT + v) * T + t], cp);
-      RH* wr = ring + (wslot * N + i) * rstride + t;
-      update_node<R, M, CF, RH>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s, buf, pm, cp, gf, c, bad,
-                                [&](int ch, R v) { wr[ch * T] = (RH)v; });
-    }
+    for (int i = tid / T; i < N; i += nthr / T) b_item(i, cbuf, k, s, buf, wslot);
     // bookkeeping of step k+1 into the other buffer (read after the next
     // phase A barrier; its previous readers finished before this step's)
     if (tid == 0)

@@ -127,7 +127,7 @@ struct tvbk_plan {
   std::vector<int64_t> remaining;     // per instance (-1 = unlimited)
   std::vector<ConnDesc> conns;
   // LDS layout
-  int lds_bold = 0, lds_state = 0, lds_c = 0, lds_edge = 0, lds_w = 0, lds_rp = 0, lds_total = 0;
+  int lds_bold = 0, lds_state = 0, lds_c = 0, lds_edge = 0, lds_w = 0, lds_rp = 0, lds_par = 0, lds_total = 0;
   int64_t img_bytes = 0;
   bool csr_in_lds = false;
   bool uniform_rows = false;   // batch-lane, n_channel * T a multiple of 32
@@ -259,6 +259,8 @@ static int layout_lds(tvbk_plan* p, int T, int64_t max_lds, bool allow_csr) {
     }
   off = align_up(off, 16);
   p->img_bytes = off;
+  p->lds_par = (int)off;                       // per-lane parameters [prow][T]
+  off = align_up(off + (int64_t)p->prow * T * p->esize, 16);
   p->lds_c = (int)off;
   off += 2 * (int64_t)N * p->nct * T * p->hsize;
   off = align_up(off, 16);
@@ -958,6 +960,7 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
   a.lds_bold = p->lds_bold;
   a.lds_state = p->lds_state;
   a.lds_c = p->lds_c;
+  a.lds_par = p->lds_par;
   a.lds_edge = p->lds_edge;
   a.lds_w = p->lds_w;
   a.lds_rp = p->lds_rp;

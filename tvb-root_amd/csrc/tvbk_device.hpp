@@ -102,11 +102,14 @@ template <class M> struct Cfun<M, CF_LINEAR> {
   static constexpr int NCH = M::NCV, NCT = M::NCV, NCP = 2;
   template <class R> __device__ static inline R channel(const R* x, int ch, const R* cp) { return x[M::cvar(ch)]; }
   template <class R> __device__ static inline R post(R gx, const R* cp) { return cp[0] * gx + cp[1]; }
+  // parameters strided in LDS
+  template <class R> __device__ static inline R post(R gx, const R* cp, int st) { return cp[0] * gx + cp[st]; }
 };
 template <class M> struct Cfun<M, CF_IDENTITY> {
   static constexpr int NCH = M::NCV, NCT = M::NCV, NCP = 0;
   template <class R> __device__ static inline R channel(const R* x, int ch, const R* cp) { return x[M::cvar(ch)]; }
   template <class R> __device__ static inline R post(R gx, const R* cp) { return gx; }
+  template <class R> __device__ static inline R post(R gx, const R* cp, int st) { return gx; }
 };
 template <class M> struct Cfun<M, CF_SIGMOID_JR> {
   static constexpr int NCH = 1, NCT = 1, NCP = 5;
@@ -115,6 +118,7 @@ template <class M> struct Cfun<M, CF_SIGMOID_JR> {
     return cp[0] + cp[1] / (R(1) + exp(cp[2] * (cp[3] - (x[M::cvar(0)] - x[M::cvar(1)]))));
   }
   template <class R> __device__ static inline R post(R gx, const R* cp) { return cp[4] * gx; }
+  template <class R> __device__ static inline R post(R gx, const R* cp, int st) { return cp[4 * st] * gx; }
 };
 
 // ---------------------------------------------------------------------------

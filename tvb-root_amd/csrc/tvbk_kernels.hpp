@@ -61,6 +61,7 @@ struct StepArgs {
   uint32_t key0, key1;
   double dt, sqrt_dt;
   int32_t lds_bold, lds_state, lds_c, lds_edge, lds_w, lds_rp;
+  int32_t lds_par;   // per-lane parameters [prow][T] (state precision)
   // dense zero-delay kernel: W padded to [16*RB][KP] (float), KB = KP/4,
   // LDS offset of the double-buffered coupling-channel array X
   const float* dense_w;

@@ -105,13 +105,28 @@ __device__ inline void step_info(const MonArgs& mo, int k, StepInfo& si) {
   si.mp = mo.qb0 + pk / mo.ii;
 }
 
+// Per-lane parameter rows (model, coupling, noise gfun; node 0's row for
+// spatial parameters) staged in LDS as [param][lane] with stride T, read by
+// phase B instead of being held in registers across the step loop.
+template <class R>
+__device__ inline void stage_params(const StepArgs& a, unsigned char* smem, int inst0, int n_inst, int N) {
+  R* s_par = reinterpret_cast<R*>(smem + a.lds_par);
+  const int T = a.T;
+  const int64_t row = a.spatial ? (int64_t)N * a.prow : (int64_t)a.prow;
+  for (int q = threadIdx.x; q < a.prow * T; q += blockDim.x) {
+    const int k = q / T, tt = q - (q / T) * T;
+    const int inst = inst0 + (tt < n_inst ? tt : 0);
+    s_par[q] = reinterpret_cast<const R*>(a.params)[(int64_t)inst * row + k];
+  }
+}
+
 // Integrator step, history write and monitors of one (node i, lane t) pair
 // with its coupling terms c (phase B of the step kernels).  `sink(ch, v)`
 // stores history channel ch of the new state.
 template <class R, class M, int CF, class NR, class SINK>
 __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, unsigned char* smem, R* s_x, int i,
                                             int t, int T, int N, bool live, int inst, uint64_t stream, int k,
-                                            int64_t s, int buf, const R* pm, const R* cp, R* gf, const R* c,
+                                            int64_t s, int buf, const R* par, int pst, const R* c,
                                             int& bad, SINK sink) {
   using C = Cfun<M, CF>;
   constexpr int NSV = M::NSV, NDP = M::NDP, NCP = C::NCP, NCH = C::NCH;
@@ -121,9 +136,15 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
   R x[NSV], xi[NSV];
 #pragma unroll
   for (int v = 0; v < NSV; ++v) x[v] = s_x[(i * NSV + v) * T + t];
-  R pp[NDP > 0 ? NDP : 1];
+  // this lane's parameters (registers, stride 1, or LDS, stride T): model,
+  // coupling, noise gfun
+  R pp[NDP > 0 ? NDP : 1], cp[NCP > 0 ? NCP : 1], gf[NSV];
 #pragma unroll
-  for (int v = 0; v < NDP; ++v) pp[v] = pm[v];
+  for (int v = 0; v < NDP; ++v) pp[v] = par[v * pst];
+#pragma unroll
+  for (int v = 0; v < NCP; ++v) cp[v] = par[(NDP + v) * pst];
+#pragma unroll
+  for (int v = 0; v < NSV; ++v) gf[v] = (a.noise != NOISE_NONE) ? par[(NDP + NCP + v) * pst] : R(0);
   if (a.spatial) {
     const R* pr = reinterpret_cast<const R*>(a.params) + (int64_t)inst * spatial_row + (int64_t)i * a.prow;
 #pragma unroll
@@ -258,7 +279,11 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
 // monitor type (float/float, double/double, or float/double -- the
 // reference's own mix).  Float64 histories get up to 8 waves per workgroup
 // and twice the registers.
-template <class RH, class R, class M, int CF>
+// PIPE = 1: the pipelined loop only (wave-uniform rows, every delay >= 1,
+// n_channel * T <= 64); PIPE = 0: the sequential loop with every gather
+// variant.  Separate instantiations keep each loop's register allocation
+// to itself.
+template <class RH, class R, class M, int CF, int PIPE>
 __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs a) {
   using C = Cfun<M, CF>;
   constexpr int NSV = M::NSV, NCH = C::NCH, NCT = C::NCT, NDP = M::NDP, NCP = C::NCP;
@@ -332,6 +357,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     }
     for (int k = tid; k <= N; k += nthr) l_rp[k] = g_rp[k];
   }
+  stage_params<R>(a, smem, td_inst0, td_n_inst, N);
   copy_bytes(img, a.img + td_img_off, a.img_bytes);
   __syncthreads();
 
@@ -340,17 +366,18 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   const int t = tid % T;
   const bool live = t < td_n_inst;
   const int inst = td_inst0 + (live ? t : 0);
-  const int64_t spatial_row = (int64_t)N * a.prow;   // per-node params: one prow row per node
-  const R* prow = reinterpret_cast<const R*>(a.params) + (int64_t)inst * (a.spatial ? spatial_row : a.prow);
-  R cp[NCP > 0 ? NCP : 1];
+  // this lane's parameters: float64 states read them from LDS (registers are
+  // short there), float32 states keep them in registers
+  constexpr bool PAR_LDS = sizeof(R) == 8;
+  constexpr int NPAR = NDP + NCP + NSV;
+  R preg[PAR_LDS ? 1 : NPAR];
+  if constexpr (!PAR_LDS) {
+    const R* prow = reinterpret_cast<const R*>(a.params) + (int64_t)inst * (a.spatial ? (int64_t)N * a.prow : a.prow);
 #pragma unroll
-  for (int k = 0; k < NCP; ++k) cp[k] = prow[NDP + k];
-  R gf[NSV];
-#pragma unroll
-  for (int k = 0; k < NSV; ++k) gf[k] = (a.noise != NOISE_NONE) ? prow[NDP + NCP + k] : R(0);
-  R pm[NDP > 0 ? NDP : 1];
-#pragma unroll
-  for (int k = 0; k < NDP; ++k) pm[k] = prow[k];
+    for (int k = 0; k < NPAR; ++k) preg[k] = prow[k];
+  }
+  const R* par = PAR_LDS ? reinterpret_cast<const R*>(smem + a.lds_par) + t : preg;
+  const int pst = PAR_LDS ? T : 1;
   const R dt = (R)a.dt;
   const uint64_t stream = a.streams ? a.streams[inst] : (uint64_t)inst;
   RH* ring = reinterpret_cast<RH*>(a.ring) + td_ring_off;
@@ -526,9 +553,9 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   auto b_item = [&](int i, const RH* cbuf, int k, int64_t s_, int buf, int wslot) {
     R c[NCT];
 #pragma unroll
-    for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(i * NCT + v) * T + t], cp);
+    for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(i * NCT + v) * T + t], par + NDP * pst, pst);
     RH* wr = ring + (wslot * N + i) * rstride + t;
-    update_node<R, M, CF, RH>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s_, buf, pm, cp, gf, c, bad,
+    update_node<R, M, CF, RH>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s_, buf, par, pst, c, bad,
                               [&](int ch, R v) { wr[ch * T] = (RH)v; });
   };
 
@@ -537,7 +564,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   // previous barrier, so each wave interleaves its rows of phase A(s+1) with
   // its updates of phase B(s) -- a row's loads are in flight while the wave
   // integrates -- and one barrier per step orders both against the next.
-  if (a.uniform_rows && !a.zero_delay && QN <= 64) {
+  if constexpr (PIPE) {
     phase_a_uni(s_c);                                   // coupling of the first step
     __syncthreads();
     const int w0 = __builtin_amdgcn_readfirstlane(wave);
@@ -565,7 +592,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
       smod = wslot;
       __syncthreads();
     }
-  } else
+  } else {
   for (int k = 0; k < nsteps; ++k, ++s) {
     const int buf = k & 1;
     RH* cbuf = s_c + (int64_t)buf * N * NCT * T;
@@ -583,6 +610,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
       for (int m = 0; m < a.n_mon; ++m) step_info(kc.mon[m], k + 1, kc.si[buf ^ 1][m]);
     smod = wslot;
     if (a.zero_delay) __syncthreads();
+  }
   }
   __syncthreads();
   copy_bytes(a.img + td_img_off, img, a.img_bytes);
@@ -648,6 +676,7 @@ __global__ void __launch_bounds__(1024) k_dense(StepArgs a) {
   R* s_x = reinterpret_cast<R*>(smem + a.lds_state);
   RH* X = reinterpret_cast<RH*>(smem + a.lds_x);   // [2][NCH][KP][T]
   RH* ring = reinterpret_cast<RH*>(a.ring) + td_ring_off;   // one slot: [node][ch][T]
+  stage_params<R>(a, smem, td_inst0, td_n_inst, N);
   copy_bytes(img, a.img + td_img_off, a.img_bytes);
   for (int q = tid; q < 2 * NCH * KP * T; q += nthr) {
     const int tt = q % T, kk = (q / T) % KP, ch = (q / (T * KP)) % NCH, b = q / (T * KP * NCH);
@@ -663,16 +692,16 @@ __global__ void __launch_bounds__(1024) k_dense(StepArgs a) {
 
   const bool live = t < td_n_inst;
   const int inst = td_inst0 + (live ? t : 0);
-  const R* prow = reinterpret_cast<const R*>(a.params) + (int64_t)inst * a.prow;
-  R cp[NCP > 0 ? NCP : 1];
+  constexpr bool PAR_LDS = sizeof(R) == 8;
+  constexpr int NPAR = NDP + NCP + NSV;
+  R preg[PAR_LDS ? 1 : NPAR];
+  if constexpr (!PAR_LDS) {
+    const R* prow = reinterpret_cast<const R*>(a.params) + (int64_t)inst * a.prow;
 #pragma unroll
-  for (int k = 0; k < NCP; ++k) cp[k] = prow[NDP + k];
-  R gf[NSV];
-#pragma unroll
-  for (int k = 0; k < NSV; ++k) gf[k] = (a.noise != NOISE_NONE) ? prow[NDP + NCP + k] : R(0);
-  R pm[NDP > 0 ? NDP : 1];
-#pragma unroll
-  for (int k = 0; k < NDP; ++k) pm[k] = prow[k];
+    for (int k = 0; k < NPAR; ++k) preg[k] = prow[k];
+  }
+  const R* par = PAR_LDS ? reinterpret_cast<const R*>(smem + a.lds_par) + t : preg;
+  const int pst = PAR_LDS ? T : 1;
   const uint64_t stream = a.streams ? a.streams[inst] : (uint64_t)inst;
   int bad = 0;
   int64_t s = td_step0;
@@ -698,8 +727,8 @@ __global__ void __launch_bounds__(1024) k_dense(StepArgs a) {
       if (i < N) {
         R c[NCH];
 #pragma unroll
-        for (int ch = 0; ch < NCH; ++ch) c[ch] = C::post((R)d[ch][v], cp);
-        update_node<R, M, CF, RH>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s, buf, pm, cp, gf, c, bad,
+        for (int ch = 0; ch < NCH; ++ch) c[ch] = C::post((R)d[ch][v], par + NDP * pst, pst);
+        update_node<R, M, CF, RH>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s, buf, par, pst, c, bad,
                                   [&](int ch, R val) { Xn[((int64_t)ch * KP + i) * T + t] = (RH)val; });
       }
     }
@@ -763,7 +792,8 @@ __global__ void k_init_uniform(InitArgs a) {
 // ---------------------------------------------------------------------------
 template <class RH, class R, class M, int CF>
 hipError_t launch_step_t(const StepArgs& a, int ntiles, int nthreads, size_t lds, hipStream_t st) {
-  auto kern = k_step<RH, R, M, CF>;
+  const bool pipe = a.uniform_rows && !a.zero_delay && Cfun<M, CF>::NCH * a.T <= 64;
+  auto kern = pipe ? k_step<RH, R, M, CF, 1> : k_step<RH, R, M, CF, 0>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;

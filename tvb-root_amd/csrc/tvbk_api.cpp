@@ -131,6 +131,7 @@ struct tvbk_plan {
   int64_t img_bytes = 0;
   bool csr_in_lds = false;
   bool uniform_rows = false;   // batch-lane, n_channel * T a multiple of 32
+  size_t srec_w_off = 0, srec_rp_off = 0;   // weights' and row starts' word offsets in d_srec
   int mon_lds_off[TVBK_MAX_MON] = {0};
   int mon_Q[TVBK_MAX_MON] = {0};
   // device buffers
@@ -555,7 +556,8 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
                  (w64.size() * p->hsize + 16) + prm.size() * p->esize + (d.inst_stream ? (size_t)B * 8 : 0) +
                  (size_t)ring_off * p->hsize + (size_t)img_off + (size_t)bacc_off * 8 + (size_t)B * 4 +
                  p->tiles.size() * sizeof(TileDesc) +
-                 (p->uniform_rows ? (size_t)(p->conns[0].nnz + TVBK_ROW_PAD) * 16 : 0);
+                 (p->uniform_rows ? (size_t)(p->conns[0].nnz + 4 * p->conns[0].n_node + TVBK_ROW_PAD) * (8 + p->hsize) +
+                                        (size_t)(p->conns[0].n_node + 1) * 4 + 64 : 0);
     for (int m = 0; m < d.n_mon; ++m) {
       if (d.mon[m].kind == TVBK_MON_BOLD) tot += p->taps_host[m].size() * 8;
       const int64_t cap = d.max_steps / d.mon[m].istep + 1;
@@ -607,31 +609,43 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
     if ((rc = up(&p->d_streams, d.inst_stream, (size_t)B * 8))) return bail(rc);
   }
   std::vector<uint32_t> srec;   // host source of an async upload: alive until the final sync
+  size_t srec_w_off = 0, srec_rp_off = 0;   // word offsets of the weights and row starts
   if (p->uniform_rows) {
-    // {delay * slot bytes, source * node bytes, weight bits}: one record per
-    // CSR entry of the shared connectome, padding entries included
+    // offsets {delay * slot bytes, source * node bytes} and weights of the
+    // shared connectome's CSR, every row padded to a multiple of four
+    // entries with copies of its first entry (weight 0), then padding
     const ConnDesc& cd = p->conns[0];
+    const tvbk_connectome& cn = d.conns[0];
     const uint32_t cstride = (uint32_t)(p->nch * T * p->hsize), sstride = (uint32_t)cd.n_node * cstride;
-    const int64_t ne = cd.nnz + TVBK_ROW_PAD;
-    srec.assign((size_t)ne * 4, 0u);
-    for (int64_t e = 0; e < ne; ++e) {
-      const uint32_t ed = edges[cd.e_off + e];
-      uint32_t* r = &srec[(size_t)e * 4];
-      r[0] = (ed >> 16) * sstride;
-      r[1] = (ed & 0xffffu) * cstride;
-      if (p->hsize == 8) {
-        uint64_t bits;
-        std::memcpy(&bits, &w64[cd.e_off + e], 8);
-        r[2] = (uint32_t)bits;
-        r[3] = (uint32_t)(bits >> 32);
-      } else {
-        const float wf = (float)w64[cd.e_off + e];
-        std::memcpy(&r[2], &wf, 4);
-        r[3] = 0;
+    std::vector<int32_t> prp(cd.n_node + 1, 0);
+    for (int i = 0; i < cd.n_node; ++i) prp[i + 1] = prp[i] + (cn.row_ptr[i + 1] - cn.row_ptr[i] + 3) / 4 * 4;
+    const int64_t ne = prp[cd.n_node] + TVBK_ROW_PAD;
+    const size_t wpe = (size_t)p->hsize / 4;
+    srec_w_off = align_up((int64_t)ne * 2, 4);
+    srec_rp_off = align_up((int64_t)(srec_w_off + (size_t)ne * wpe), 4);
+    srec.assign(srec_rp_off + cd.n_node + 1, 0u);
+    for (int i = 0; i < cd.n_node; ++i) {
+      const int r0 = cn.row_ptr[i], n = cn.row_ptr[i + 1] - r0;
+      for (int k = 0; k < prp[i + 1] - prp[i]; ++k) {
+        const int64_t src = cd.e_off + r0 + (k < n ? k : 0);
+        const uint32_t ed = edges[src];
+        const size_t e = (size_t)prp[i] + k;
+        srec[e * 2] = (ed >> 16) * sstride;
+        srec[e * 2 + 1] = (ed & 0xffffu) * cstride;
+        const double wv = k < n ? w64[src] : 0.0;
+        if (p->hsize == 8) {
+          std::memcpy(&srec[srec_w_off + e * 2], &wv, 8);
+        } else {
+          const float wf = (float)wv;
+          std::memcpy(&srec[srec_w_off + e], &wf, 4);
+        }
       }
     }
+    std::memcpy(&srec[srec_rp_off], prp.data(), prp.size() * 4);
     if ((rc = up(&p->d_srec, srec.data(), srec.size() * 4))) return bail(rc);
   }
+  p->srec_rp_off = srec_rp_off;
+  p->srec_w_off = srec_w_off;
   if (kernel == TVBK_KERNEL_DENSE_MFMA) {
     const tvbk_connectome& cn = d.conns[0];
     const int KP = 4 * p->dense_kb, NR = 16 * p->dense_rb;
@@ -948,7 +962,9 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
   a.scheme = d.scheme;
   a.noise = d.noise;
   a.csr_in_lds = p->csr_in_lds ? 1 : 0;
-  a.srec = (const uint4*)p->d_srec.p;
+  a.sofs = (const uint32_t*)p->d_srec.p;
+  a.swts = p->d_srec.p ? (const void*)((const uint32_t*)p->d_srec.p + p->srec_w_off) : nullptr;
+  a.srp = p->d_srec.p ? (const int32_t*)((const uint32_t*)p->d_srec.p + p->srec_rp_off) : nullptr;
   a.uniform_rows = p->uniform_rows ? 1 : 0;
   int zero_delay = 0;
   for (const ConnDesc& cd : p->conns) zero_delay |= (cd.d_min == 0 && cd.nnz > 0);

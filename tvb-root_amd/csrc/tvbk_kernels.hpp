@@ -45,9 +45,12 @@ struct StepArgs {
   const uint32_t* edges;
   const void* weights;
   const int32_t* row_ptr;
-  // per-plan edge records for the wave-uniform row path: {delay * slot
-  // bytes, source * node bytes, weight bits lo, hi} (one shared connectome)
-  const uint4* srec;
+  // per-plan edge arrays of the wave-uniform row path (one shared
+  // connectome): offsets {delay * slot bytes, source * node bytes} and the
+  // weights in the history precision
+  const uint32_t* sofs;
+  const void* swts;
+  const int32_t* srp;   // row starts in those arrays (rows padded to multiples of 4)
   void* ring;
   unsigned char* img;
   double* bacc;

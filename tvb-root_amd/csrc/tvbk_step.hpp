@@ -71,16 +71,18 @@ template <> __device__ inline double rec_weight<double>(const EdgeRec& e) {
   return __hiloint2double((int)e.w1, (int)e.w0);
 }
 
-// Edge records of the wave-uniform row path, read through the scalar cache:
-// {delay * slot bytes, source node * node bytes, weight bits lo, hi}.
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-typedef const __attribute__((address_space(4))) u32x4 ScalarRec;
+// Edge arrays of the wave-uniform row path, read through the scalar cache
+// in 32-byte chunks: offsets {delay * slot bytes, source node * node bytes}
+// (four edges a chunk) and weights (eight float32 or four float64 a chunk).
+typedef unsigned int u32x8a __attribute__((ext_vector_type(8), aligned(4)));
+typedef const __attribute__((address_space(4))) u32x8a ScalarChunk;
+typedef const __attribute__((address_space(4))) uint32_t ScalarU32;
 typedef const __attribute__((address_space(4))) int32_t ScalarInt;
 
-template <class R> __device__ inline R srec_weight(u32x4 r);
-template <> __device__ inline float srec_weight<float>(u32x4 r) { return __uint_as_float(r.z); }
-template <> __device__ inline double srec_weight<double>(u32x4 r) {
-  return __hiloint2double((int)r.w, (int)r.z);
+template <class R> __device__ inline R chunk_weight(const u32x8a& c, int j);
+template <> __device__ inline float chunk_weight<float>(const u32x8a& c, int j) { return __uint_as_float(c[j]); }
+template <> __device__ inline double chunk_weight<double>(const u32x8a& c, int j) {
+  return __hiloint2double((int)c[2 * j + 1], (int)c[2 * j]);
 }
 
 // history value at vector offset voff (per lane) + scalar offset soff
@@ -395,6 +397,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   // slot*sstride + col*cstride + lane_off.  Instantiated once for the LDS and
   // once for the global CSR so every access has a known address space.
   constexpr int RMAX = sizeof(RH) == 4 ? 32 : 16;
+  constexpr int WPE = sizeof(RH) / 4;                   // 32-bit words per weight
   const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(ring, (uint32_t)H * sstride);
   const int rmax = cd_max_row < RMAX ? cd_max_row : RMAX;
   auto slot_off = [&](uint32_t delay) -> uint32_t {
@@ -461,39 +464,40 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   // edge records come through the scalar cache and each edge's history
   // offset is a scalar buffer offset, so an edge costs each lane one load,
   // one multiply and one add
-  ScalarRec* c_rec = (ScalarRec*)(a.srec + cd_e_off);
+  // padded row layout of the one shared connectome
+  ScalarU32* c_ofs = (ScalarU32*)a.sofs;                                  // {ds, coff} per entry
+  ScalarU32* c_w = (ScalarU32*)a.swts;                                    // weight words per entry
+  ScalarInt* c_prp = (ScalarInt*)a.srp;                                   // padded row starts
   ScalarInt* c_rp = (ScalarInt*)(a.row_ptr + cd_rp_off);
   const uint32_t Hs = (uint32_t)H * sstride;
   const uint32_t hs = __builtin_amdgcn_readfirstlane(Hs);
   const int QN = NCH * T;                               // lanes per target node
-  // history byte offset of an edge record for slot base `base` ((s-1) % H
-  // times the slot bytes); offsets stay below 2^31 (a tile's history < 2 GiB)
-  auto soff = [&](uint32_t base, u32x4 r) -> uint32_t {
-    int o = (int)base - (int)r.x;
-    o += (o < 0) ? (int)hs : 0;
-    return (uint32_t)o + r.y;
+  // history byte offset of edge {ds, coff} for slot base `base` ((s-1) % H
+  // times the slot bytes) and baseh = base + H * slot bytes: the slot
+  // difference wraps at H, min over unsigned picks the in-range one
+  auto soff = [&](uint32_t base, uint32_t baseh, uint32_t ds, uint32_t coff) -> uint32_t {
+    const uint32_t o1 = base - ds, o2 = baseh - ds;
+    return (o1 < o2 ? o1 : o2) + coff;
   };
-  // issue the loads of row i (n <= RMAX): blocks of four under one uniform
-  // branch; entries of the last block past the row's end re-read its first
-  // entry (a cache hit) and are never summed.  Returns n; rows longer than
-  // RMAX load nothing here.
+  // issue the loads of row i (n <= RMAX) in blocks of four edges.  The
+  // uniform-path arrays pad every row to a multiple of four with copies of
+  // the row's first edge (re-read, a cache hit, never summed), from row
+  // start c_prp[i]; registers of blocks past the row stay unset and unread.
+  // Returns n; rows longer than RMAX load nothing here.
   auto row_issue = [&](int i, uint32_t base, uint32_t lane_off, RH* v, int& e0) -> int {
-    e0 = __builtin_amdgcn_readfirstlane(c_rp[i]);
-    const int n = __builtin_amdgcn_readfirstlane(c_rp[i + 1]) - e0;
+    e0 = __builtin_amdgcn_readfirstlane(c_prp[i]);
+    const int n = __builtin_amdgcn_readfirstlane(c_rp[i + 1]) - __builtin_amdgcn_readfirstlane(c_rp[i]);
     if (n <= RMAX) {
-      const uint32_t so0 = soff(base, c_rec[e0]);
+      const uint32_t baseh = base + hs;
+      ScalarChunk* q = (ScalarChunk*)(c_ofs + 2 * e0);
 #pragma unroll
       for (int b = 0; b < RMAX / 4; ++b) {
         if (4 * b < n) {
+          const u32x8a r = q[b];
 #pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            const int e = 4 * b + k;
-            const uint32_t so = e < n ? soff(base, c_rec[e0 + e]) : so0;
-            v[e] = buf_load2<RH>(rsrc, lane_off, __builtin_amdgcn_readfirstlane(so));
-          }
-        } else {
-#pragma unroll
-          for (int k = 0; k < 4; ++k) v[4 * b + k] = RH(0);
+          for (int k = 0; k < 4; ++k)
+            v[4 * b + k] = buf_load2<RH>(rsrc, lane_off,
+                                         __builtin_amdgcn_readfirstlane(soff(base, baseh, r[2 * k], r[2 * k + 1])));
         }
       }
     }
@@ -501,15 +505,30 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   };
   // weights and the numpy reduceat order; long rows are loaded here
   auto row_finish = [&](int e0, int n, uint32_t base, uint32_t lane_off, RH* v) -> RH {
+    constexpr int EPC = 8 / WPE;                          // weights per 32-byte chunk
     if (n <= RMAX) {
+      ScalarChunk* wq = (ScalarChunk*)(c_w + WPE * e0);
 #pragma unroll
-      for (int e = 0; e < RMAX; ++e) v[e] = srec_weight<RH>(c_rec[e0 + e]) * v[e];
+      for (int c = 0; c < RMAX / EPC; ++c) {
+        if (c * EPC < n) {
+          const u32x8a wc = wq[c];
+#pragma unroll
+          for (int j = 0; j < EPC; ++j)
+            if ((c * EPC + j) / 4 * 4 < n) v[c * EPC + j] = chunk_weight<RH>(wc, j) * v[c * EPC + j];
+        }
+      }
       return reduceat_regs<RMAX, RH>(n, v);
     }
+    const uint32_t baseh = base + hs;
     auto ld = [&](int e) -> RH {
-      return e < n ? buf_load2<RH>(rsrc, lane_off, __builtin_amdgcn_readfirstlane(soff(base, c_rec[e0 + e]))) : RH(0);
+      if (e >= n) return RH(0);
+      const uint32_t so = soff(base, baseh, c_ofs[2 * (e0 + e)], c_ofs[2 * (e0 + e) + 1]);
+      return buf_load2<RH>(rsrc, lane_off, __builtin_amdgcn_readfirstlane(so));
     };
-    auto wt = [&](int e) -> RH { return srec_weight<RH>(c_rec[e0 + e]); };
+    auto wt = [&](int e) -> RH {
+      if constexpr (WPE == 1) return __uint_as_float(c_w[e0 + e]);
+      else return __hiloint2double((int)c_w[2 * (e0 + e) + 1], (int)c_w[2 * (e0 + e)]);
+    };
     return reduceat_row_blocked<RH>(n, ld, wt);
   };
   auto row_store = [&](RH* cbuf, int i, int q, RH acc) {
@@ -567,7 +586,9 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   if constexpr (PIPE) {
     phase_a_uni(s_c);                                   // coupling of the first step
     __syncthreads();
-    const int w0 = __builtin_amdgcn_readfirstlane(wave);
+    // rows from the last wave down: the first waves carry the extra phase B
+    // pass when N * T is not a multiple of the block
+    const int w0 = __builtin_amdgcn_readfirstlane(nwave - 1 - wave);
     const int nrow = (N - w0 + nwave - 1) / nwave;      // rows of this wave
     const int npass = (N + nthr / T - 1) / (nthr / T);  // phase B passes
     const int nu = nrow > npass ? nrow : npass;

@@ -74,8 +74,9 @@ def box_muller(words, precision="f64"):
     f64: u1 = (w0 + 1) 2^-32 in (0, 1], u2 = w1 2^-32 in [0, 1),
          z0 = sqrt(-2 ln u1) cos(2 pi u2), z1 = ... sin(2 pi u2); same for (w2, w3).
     f32: the same with 24-bit uniforms u1 = ((w0 >> 8) + 1) 2^-24,
-         u2 = (w1 >> 8) 2^-24; returned in float64 (the device rounds to float32
-         through its own float32 log/sin/cos)."""
+         u2 = (w1 >> 8) 2^-24; returned in float64 (the device evaluates it in
+         float32 with the hardware log2, sqrt, sin and cos instructions, a few
+         float32 ulp from these values)."""
     w = words.astype(np.float64) if precision == "f64" else (words >> np.uint32(8)).astype(np.float64)
     scale = 2.0 ** -32 if precision == "f64" else 2.0 ** -24
     out = np.empty(words.shape, np.float64)

@@ -289,8 +289,10 @@ __device__ inline U4 philox10(U4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
     if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-    const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
-    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+    // one 32 x 32 -> 64-bit product gives both halves (v_mad_u64_u32)
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c.x, p1 = (uint64_t)0xCD9E8D57u * c.z;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
     c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
   }
   return c;
@@ -306,14 +308,15 @@ __device__ inline void box_muller(uint32_t a, uint32_t b, double* z0, double* z1
   *z1 = rad * s;
 }
 
+// float32: the hardware transcendentals -- v_log_f32 (log2), v_sqrt_f32 and
+// v_sin_f32 / v_cos_f32, which take the angle in revolutions, so
+// sin(2 pi u2) is one instruction; each within a few ulp.
 __device__ inline void box_muller(uint32_t a, uint32_t b, float* z0, float* z1) {
-  const float u1 = (float)((a >> 8) + 1u) * 5.9604644775390625e-8f;  // 2^-24
-  const float u2 = (float)(b >> 8) * 5.9604644775390625e-8f;
-  const float rad = sqrtf(-2.0f * logf(u1));
-  float s, c;
-  sincosf(6.2831853071795865f * u2, &s, &c);
-  *z0 = rad * c;
-  *z1 = rad * s;
+  const float u1 = (float)((a >> 8) + 1u) * 5.9604644775390625e-8f;  // 2^-24, in (0, 1]
+  const float u2 = (float)(b >> 8) * 5.9604644775390625e-8f;         // [0, 1)
+  const float rad = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));   // -2 ln u1
+  *z0 = rad * __builtin_amdgcn_cosf(u2);
+  *z1 = rad * __builtin_amdgcn_sinf(u2);
 }
 
 // Standard normals for state variables 0..nsv-1 of (node, stream, step).

@@ -181,7 +181,7 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
 
   // ---------------- monitors (monitors.py:160-190, :363-391, :834-962)
   for (int m = 0; m < a.n_mon; ++m) {
-    const MonArgs& mo = kc.mon[m];
+    const MonArgs& mo = a.mon[m];   // plan constants: kernel-argument (scalar) loads
     const StepInfo& si = kc.si[buf][m];
     const int nv = mo.nvoi;
     const bool emit = si.emit != 0;
@@ -192,7 +192,7 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
       for (int v = 0; v < nv; ++v) {
         const int o = mo.voi[v];
         R val;
-        const int osv = kc.obs_sv[o];
+        const int osv = a.obs_sv[o];
         if (osv >= 0) {
           val = x[0];
 #pragma unroll
@@ -200,7 +200,7 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
         } else {
           val = R(0);
 #pragma unroll
-          for (int u = 0; u < NSV; ++u) val = val + (R)kc.obs_coef[o][u] * x[u];
+          for (int u = 0; u < NSV; ++u) val = val + (R)a.obs_coef[o][u] * x[u];
         }
         if (mo.kind == MON_RAW) {
           if (live) out[v * td.out_sv[m]] = val;
@@ -235,7 +235,7 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
       for (int v = 0; v < nv; ++v) {
         const int o = mo.voi[v];
         R val;
-        const int osv = kc.obs_sv[o];
+        const int osv = a.obs_sv[o];
         if (osv >= 0) {
           val = x[0];
 #pragma unroll
@@ -243,7 +243,7 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
         } else {
           val = R(0);
 #pragma unroll
-          for (int u = 0; u < NSV; ++u) val = val + (R)kc.obs_coef[o][u] * x[u];
+          for (int u = 0; u < NSV; ++u) val = val + (R)a.obs_coef[o][u] * x[u];
         }
         double* bsum = bi + (i * nv + v) * T + t;
         *bsum = *bsum + (double)val;

@@ -195,6 +195,44 @@ __device__ inline R reduceat_regs(int n, const R* v) {
   return v[0] + res;
 }
 
+// The same order with the products formed on use: x_e = w_e * v_e (rounded,
+// as numpy multiplies before it sums); only entries e < n are touched.  The
+// tail after the blocks of eight is selected, not branched, per entry.
+template <int RMAX, class R, class W>
+__device__ inline R reduceat_regs_w(int n, const R* v, const W* w) {
+  auto x = [&](int e) -> R { return (R)w[e] * v[e]; };
+  if (n <= 1) return n == 1 ? x(0) : R(0);
+  const int m = n - 1;                       // pairwise part x[1..n-1]
+  R res;
+  if (m < 8) {
+    res = R(0);
+#pragma unroll
+    for (int i = 0; i < 7; ++i)
+      if (i < m) res += x(1 + i);
+  } else {
+    const int nfull = m - (m & 7), rem = m & 7;
+    R r[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) r[k] = x(1 + k);
+#pragma unroll
+    for (int b = 8; b + 8 <= RMAX - 1; b += 8)
+      if (b < nfull) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) r[k] += x(1 + b + k);
+      }
+    res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    // tail x[1 + nfull .. m - 1]: one uniform branch per possible nfull
+#pragma unroll
+    for (int b = 8; b < RMAX - 1; b += 8)
+      if (nfull == b) {
+#pragma unroll
+        for (int j = 0; j < 7 && b + j < RMAX - 1; ++j)
+          if (j < rem) res += x(1 + b + j);
+      }
+  }
+  return x(0) + res;
+}
+
 // numpy add.reduceat order for rows longer than the register path: the
 // pairwise blocks of eight are loaded as a batch before they are used.
 // ld(e) returns the raw value of entry e (zero past the row end), wt(e) its

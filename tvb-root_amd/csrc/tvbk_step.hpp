@@ -29,6 +29,8 @@
 //   the persistent part of LDS (bold interim, state, tavg) is copied from and
 //   back to a per-tile image in HBM around each call.
 #pragma once
+#include <type_traits>
+
 #include "tvbk_kernels.hpp"
 #include "tvbk_device.hpp"
 
@@ -398,6 +400,10 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   // once for the global CSR so every access has a known address space.
   constexpr int RMAX = sizeof(RH) == 4 ? 32 : 16;
   constexpr int WPE = sizeof(RH) / 4;                   // 32-bit words per weight
+  using RMc = std::integral_constant<int, RMAX>;
+  // the pipelined loop keeps two rows of RP registers in flight
+  constexpr int RP = sizeof(RH) == 4 ? 24 : 12;
+  using RPc = std::integral_constant<int, RP>;
   const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(ring, (uint32_t)H * sstride);
   const int rmax = cd_max_row < RMAX ? cd_max_row : RMAX;
   auto slot_off = [&](uint32_t delay) -> uint32_t {
@@ -484,14 +490,16 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   // the row's first edge (re-read, a cache hit, never summed), from row
   // start c_prp[i]; registers of blocks past the row stay unset and unread.
   // Returns n; rows longer than RMAX load nothing here.
-  auto row_issue = [&](int i, uint32_t base, uint32_t lane_off, RH* v, int& e0) -> int {
+  // rm: std::integral_constant with the register row length RM
+  auto row_issue = [&](auto rm, int i, uint32_t base, uint32_t lane_off, RH* v, int& e0) -> int {
+    constexpr int RM = decltype(rm)::value;
     e0 = __builtin_amdgcn_readfirstlane(c_prp[i]);
     const int n = __builtin_amdgcn_readfirstlane(c_rp[i + 1]) - __builtin_amdgcn_readfirstlane(c_rp[i]);
-    if (n <= RMAX) {
+    if (n <= RM) {
       const uint32_t baseh = base + hs;
       ScalarChunk* q = (ScalarChunk*)(c_ofs + 2 * e0);
 #pragma unroll
-      for (int b = 0; b < RMAX / 4; ++b) {
+      for (int b = 0; b < RM / 4; ++b) {
         if (4 * b < n) {
           const u32x8a r = q[b];
 #pragma unroll
@@ -504,20 +512,23 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     return n;
   };
   // weights and the numpy reduceat order; long rows are loaded here
-  auto row_finish = [&](int e0, int n, uint32_t base, uint32_t lane_off, RH* v) -> RH {
+  auto row_finish = [&](auto rm, int e0, int n, uint32_t base, uint32_t lane_off, RH* v) -> RH {
+    constexpr int RM = decltype(rm)::value;
     constexpr int EPC = 8 / WPE;                          // weights per 32-byte chunk
-    if (n <= RMAX) {
+    if (n <= RM) {
+      // weights in 32-byte scalar chunks; the products are formed inside the
+      // reduction, on entries < n only
       ScalarChunk* wq = (ScalarChunk*)(c_w + WPE * e0);
+      RH w[RM];
 #pragma unroll
-      for (int c = 0; c < RMAX / EPC; ++c) {
+      for (int c = 0; c < RM / EPC; ++c) {
         if (c * EPC < n) {
           const u32x8a wc = wq[c];
 #pragma unroll
-          for (int j = 0; j < EPC; ++j)
-            if ((c * EPC + j) / 4 * 4 < n) v[c * EPC + j] = chunk_weight<RH>(wc, j) * v[c * EPC + j];
+          for (int j = 0; j < EPC; ++j) w[c * EPC + j] = chunk_weight<RH>(wc, j);
         }
       }
-      return reduceat_regs<RMAX, RH>(n, v);
+      return reduceat_regs_w<RM, RH>(n, v, w);
     }
     const uint32_t baseh = base + hs;
     auto ld = [&](int e) -> RH {
@@ -546,8 +557,8 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
       for (int i = __builtin_amdgcn_readfirstlane(wave); i < N; i += nwave) {   // i uniform: scalar loads
         RH v[RMAX];
         int e0;
-        const int n = row_issue(i, base, lane_off, v, e0);
-        row_store(cbuf, i, q, row_finish(e0, n, base, lane_off, v));
+        const int n = row_issue(RMc{}, i, base, lane_off, v, e0);
+        row_store(cbuf, i, q, row_finish(RMc{}, e0, n, base, lane_off, v));
       }
     }
   };
@@ -599,14 +610,20 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
       RH* cnext = s_c + (int64_t)(buf ^ 1) * N * NCT * T;
       const int wslot = (smod + 1 == H) ? 0 : smod + 1;   // s % H
       const uint32_t base_n = __builtin_amdgcn_readfirstlane((uint32_t)wslot * sstride);   // (s+1-1) % H
-      for (int u = 0; u < nu; ++u) {
-        const int ra = w0 + u * nwave;                  // uniform
+      // rows u and u+1 in flight: row u+1 is issued, then this wave's
+      // update u runs, then row u is reduced; va / vb alternate
+      RH va[RP], vb[RP];
+      int ea = 0, na = 0, eb = 0, nb = 0;
+      if (nrow > 0) na = row_issue(RPc{}, w0, base_n, lane_off, va, ea);
+      auto body = [&](int u, RH* vf, int ef, int nf, RH* vi, int& ei, int& ni) {
+        if (u + 1 < nrow) ni = row_issue(RPc{}, w0 + (u + 1) * nwave, base_n, lane_off, vi, ei);
         const int ib = tid / T + u * (nthr / T);
-        RH v[RMAX];
-        int e0 = 0, n = 0;
-        if (ra < N) n = row_issue(ra, base_n, lane_off, v, e0);
         if (ib < N) b_item(ib, cbuf, k, s, buf, wslot);
-        if (ra < N) row_store(cnext, ra, lane, row_finish(e0, n, base_n, lane_off, v));
+        if (u < nrow) row_store(cnext, w0 + u * nwave, lane, row_finish(RPc{}, ef, nf, base_n, lane_off, vf));
+      };
+      for (int u = 0; u < nu; u += 2) {
+        body(u, va, ea, na, vb, eb, nb);
+        if (u + 1 < nu) body(u + 1, vb, eb, nb, va, ea, na);
       }
       if (tid == 0)
         for (int m = 0; m < a.n_mon; ++m) step_info(kc.mon[m], k + 1, kc.si[buf ^ 1][m]);

@@ -998,6 +998,7 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
     ma.nvoi = mo.n_voi;
     for (int v = 0; v < mo.n_voi; ++v) ma.voi[v] = mo.voi[v];
     ma.ii = mo.kind == TVBK_MON_BOLD ? mo.interim_istep : 1;
+    ma.rat = (mo.kind == TVBK_MON_BOLD && mo.istep % ma.ii == 0) ? mo.istep / ma.ii : 0;
     ma.S = mo.n_taps;
     ma.Q = p->mon_Q[m] > 0 ? p->mon_Q[m] : 1;
     ma.sub1 = mo.subtract_one;

@@ -30,6 +30,7 @@ struct TileDesc {
 
 struct MonArgs {
   int32_t kind, istep, nvoi, ii, S, Q, sub1, lds_off;
+  int32_t rat, pad;   // Bold: istep / ii when ii divides istep, else 0
   int32_t voi[TVBK_KMAX_OBS];
   double k1v0;
   const double* taps;

@@ -51,6 +51,10 @@ __device__ inline void copy_bytes(unsigned char* dst, const unsigned char* src, 
 struct StepInfo {
   int32_t emit, push, msample, pad;
   int64_t mp, kk;
+  // Bold push with istep = rat * ii: emissions klo .. klo + fir_n - 1 take the
+  // interim sample, at accumulator slots fir_slot0 + j (mod Q) with ages
+  // fir_age0 + j * rat
+  int32_t fir_n, fir_slot0, fir_age0, pad2;
 };
 
 struct KCfg {
@@ -107,6 +111,13 @@ __device__ inline void step_info(const MonArgs& mo, int k, StepInfo& si) {
   const int pk = mo.rb0 + k;
   si.push = (pk % mo.ii) == 0;
   si.mp = mo.qb0 + pk / mo.ii;
+  if (mo.kind == MON_BOLD && mo.rat > 0 && si.push) {
+    const int64_t klo = (si.mp + mo.rat - 1) / mo.rat;
+    const int64_t khi = (si.mp + mo.S - 1) / mo.rat;
+    si.fir_n = (int32_t)(khi - klo + 1);
+    si.fir_slot0 = (int32_t)(klo % mo.Q);
+    si.fir_age0 = (int32_t)(klo * mo.rat - si.mp);
+  }
 }
 
 // Per-lane parameter rows (model, coupling, noise gfun; node 0's row for
@@ -134,7 +145,8 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
                                             int& bad, SINK sink) {
   using C = Cfun<M, CF>;
   constexpr int NSV = M::NSV, NDP = M::NDP, NCP = C::NCP, NCH = C::NCH;
-  const TileDesc& td = kc.td;
+  // this tile's descriptor: read-only during the kernel, scalar loads
+  const auto& td = *(const __attribute__((address_space(4))) TileDesc*)(a.tiles + blockIdx.x);
   const R dt = (R)a.dt;
   const int64_t spatial_row = (int64_t)N * a.prow;
   R x[NSV], xi[NSV];
@@ -188,8 +200,10 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
     const int nv = mo.nvoi;
     const bool emit = si.emit != 0;
     if (mo.kind == MON_RAW || mo.kind == MON_TAVG) {
-      R* out = reinterpret_cast<R*>(a.out[m]) + td.out_off[m] + (int64_t)si.msample * td.out_sj[m] +
+      auto outp = [&]() -> R* {
+        return reinterpret_cast<R*>(a.out[m]) + td.out_off[m] + (int64_t)si.msample * td.out_sj[m] +
                (int64_t)i * td.out_si[m] + t;
+      };
       R* ts = reinterpret_cast<R*>(smem + mo.lds_off);
       for (int v = 0; v < nv; ++v) {
         const int o = mo.voi[v];
@@ -205,7 +219,7 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
           for (int u = 0; u < NSV; ++u) val = val + (R)a.obs_coef[o][u] * x[u];
         }
         if (mo.kind == MON_RAW) {
-          if (live) out[v * td.out_sv[m]] = val;
+          if (live) outp()[v * td.out_sv[m]] = val;
         } else {
           // running sum over the averaging window: Kahan-compensated in
           // float32 (sum, compensation), plain sequential (numpy's mean
@@ -223,7 +237,7 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
           }
           *sum = tv;
           if (emit) {
-            if (live) out[v * td.out_sv[m]] = tv / (R)mo.istep;
+            if (live) outp()[v * td.out_sv[m]] = tv / (R)mo.istep;
             *sum = R(0);
             if constexpr (sizeof(R) == 4) *cmp = R(0);
           }
@@ -255,13 +269,26 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
           // sample mp contributes to emissions kk with kk*istep >= mp*ii
           // and kk*istep < (mp+S)*ii; weight by age a = floor(kk*istep/ii) - mp:
           // a == 0 -> G[S-1], else G[a-1]
-          const int64_t klo = (mp * mo.ii + mo.istep - 1) / mo.istep;
-          const int64_t khi = ((mp + mo.S) * mo.ii - 1) / mo.istep;
-          for (int64_t kk = klo; kk <= khi; ++kk) {
-            const int64_t age = (kk * mo.istep) / mo.ii - mp;
-            const double w = mo.taps[age == 0 ? mo.S - 1 : age - 1];
-            double* ap = acc + (((kk % mo.Q) * nv + v) * (int64_t)N + i) * T + t;
-            *ap = *ap + w * mean;
+          if (mo.rat > 0) {
+            // istep = rat * ii: the schedule of this push, from the step's
+            // bookkeeping (consecutive slots, ages rat apart)
+            int slot = si.fir_slot0, age = si.fir_age0;
+            for (int j = 0; j < si.fir_n; ++j) {
+              const double w = mo.taps[age == 0 ? mo.S - 1 : age - 1];
+              double* ap = acc + (((int64_t)slot * nv + v) * N + i) * T + t;
+              *ap = *ap + w * mean;
+              age += mo.rat;
+              slot = (slot + 1 == mo.Q) ? 0 : slot + 1;
+            }
+          } else {
+            const int64_t klo = (mp * mo.ii + mo.istep - 1) / mo.istep;
+            const int64_t khi = ((mp + mo.S) * mo.ii - 1) / mo.istep;
+            for (int64_t kk = klo; kk <= khi; ++kk) {
+              const int64_t age = (kk * mo.istep) / mo.ii - mp;
+              const double w = mo.taps[age == 0 ? mo.S - 1 : age - 1];
+              double* ap = acc + (((kk % mo.Q) * nv + v) * (int64_t)N + i) * T + t;
+              *ap = *ap + w * mean;
+            }
           }
         }
         if (emit) {

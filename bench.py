@@ -311,7 +311,7 @@ def main():
     ap.add_argument("--instances", type=int, default=8192)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=40000)
+    ap.add_argument("--cpu-steps", type=int, default=200000, help="oracle sample length (about 10-20 s of one host core)")
     ap.add_argument("--no-extras", action="store_true", help="skip CPU baseline, FP64 single path and error sample")
     ap.add_argument("--workload", default="c2", choices=("c2", "c2z", "c3", "c4", "copy"))
     ap.add_argument("--precision", default="mixed", choices=("f32", "mixed"),

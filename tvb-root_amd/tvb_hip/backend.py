@@ -263,7 +263,7 @@ class HipBackend:
         return out
 
     # ---------------------------------------------------------------- batches
-    def build_batch_plan(self, sims, precision="fp32", noise="inject", max_steps=65536, kernel="auto",
+    def build_batch_plan(self, sims, precision="auto", noise="inject", max_steps=65536, kernel="auto",
                          tile=0, waves=0, seed=0, streams=None, stream=None):
         """One Plan over several configured simulators of the same model,
         coupling, integrator and monitors.  Simulators whose connectivity has
@@ -310,7 +310,7 @@ class HipBackend:
             plan.set_history(b, hist, state, s.current_step)
         return plan
 
-    def run_batch(self, sims, nstep, precision="fp32", noise="inject", steps=None, kernel="auto", tile=0,
+    def run_batch(self, sims, nstep, precision="auto", noise="inject", steps=None, kernel="auto", tile=0,
                   seed=0, streams=None):
         """Run several simulators in one launch.  ``nstep``: steps of the
         longest; ``steps``: per-simulator step counts (uneven durations).
@@ -344,7 +344,7 @@ class HipBackend:
             plan.close()
         return out
 
-    def build_sweep_plan(self, sim, overrides, precision="fp32", noise="philox", max_steps=65536,
+    def build_sweep_plan(self, sim, overrides, precision="auto", noise="philox", max_steps=65536,
                          kernel="auto", tile=0, waves=0, seed=0, init="sim", init_seed=1, stream=None):
         """A Plan of B instances of one simulator with per-instance parameter
         overrides, e.g. {"coupling.a": G, "noise.nsig": sigma, "model.eta": eta}

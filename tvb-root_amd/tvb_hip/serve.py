@@ -173,7 +173,7 @@ def _concat_chunks(chunks, n_inst, n_mon):
 
 def synthetic_requests(n_requests, rate_hz, seed=0, inst_range=(1, 512), node_range=(68, 998),
                        duration_range_ms=(5000.0, 60000.0), duration_scale=1.0, models=("G2D", "MPR"),
-                       precision="fp32"):
+                       precision="mixed"):
     """Poisson stream of sweep requests over seeded synthetic connectomes:
     N log-uniform in node_range, instances uniform in inst_range, model
     chosen uniformly, HeunStochastic, TemporalAverage(1 ms), per-request
@@ -219,7 +219,7 @@ def _make_request(rid, arrival, n, B, model, dur_ms, cseed, nseed, precision):
     params[:, len(raw) + 1] = 0.0
     params[:, len(raw) + 2:] = sig[:, None]
     n_steps = int(np.ceil(dur_ms / dt))
-    prec = L.F64 if precision == "fp64" else L.F32
+    prec = {"fp64": L.F64, "fp32": L.F32, "mixed": L.MIXED}[precision]
     csr = CSR(w, d, H)
     mons = [MonitorSpec(L.MON_TAVG, int(round(1.0 / dt)), [0], name="TemporalAverage")]
     obs = np.eye(2)[:1] if model == "G2D" else np.eye(2)[:1]

@@ -258,6 +258,28 @@ int tvbk_plan_shape(tvbk_plan *plan, int32_t *kernel, int32_t *tile,
  * HIP events on the plan's stream (0 if unavailable). */
 int tvbk_last_run_ms(tvbk_plan *plan, double *ms);
 
+/* Balloon-Windkessel BOLD of neural-input series; replaces
+ * BalloonModel.evaluate (tvb_library/tvb/analyzers/fmri_balloon.py:198-275)
+ * for the "none" input transformation (:314-318) with its default
+ * HeunDeterministic scheme (integrators.py:296-318).  Device pointers:
+ *   x      input, element (t, j) at x[t * x_stride + j], t < n_samples,
+ *          j < n_series, float64 (x_elem_bytes 8) or float32 (4) -- e.g.
+ *          the first variable of a TemporalAverage output on the device;
+ *   y      output BOLD, float64 [n_samples][n_series];
+ *   state  float64 [4][n_series] (s, f, v, q), written at the end; with
+ *          start != 0 the run begins at the rest state (0, 1, 1, 1), the
+ *          first sample's BOLD taken from it (:208-213, :239-241), else it
+ *          continues from `state` and every sample advances it.
+ * params: tau_s, tau_f, tau_o, alpha, TE, V0, E0, epsilon, nu_0, r_0
+ * (:130-196).  flags: bit 0 linear BOLD equations, bit 1 classical
+ * coefficients (RBM false), bit 2 subtract each series' mean first
+ * (normalize_neural_input; needs start).  Runs on `stream` (NULL: the
+ * default stream) and returns once queued. */
+int tvbk_balloon(const void *x, int32_t x_elem_bytes, int64_t x_stride,
+                 int64_t n_samples, int64_t n_series, double sample_period_ms,
+                 const double *params, int32_t flags, int32_t start,
+                 double *state, double *y, void *stream);
+
 const char *tvbk_last_error(tvbk_plan *plan);
 
 #ifdef __cplusplus

@@ -1049,6 +1049,25 @@ int tvbk_last_run_ms(tvbk_plan* p, double* ms) {
   return TVBK_OK;
 }
 
+int tvbk_balloon(const void* x, int32_t x_elem_bytes, int64_t x_stride, int64_t n_samples, int64_t n_series,
+                 double sample_period_ms, const double* params, int32_t flags, int32_t start, double* state,
+                 double* y, void* stream) {
+  if (!x || !params || !state || !y) return fail(nullptr, TVBK_EINVAL, "null argument");
+  if (x_elem_bytes != 4 && x_elem_bytes != 8) return fail(nullptr, TVBK_EINVAL, "x_elem_bytes must be 4 or 8");
+  if (n_samples < 0 || n_series <= 0 || x_stride < n_series)
+    return fail(nullptr, TVBK_EINVAL, "bad sizes (n_series > 0, n_samples >= 0, x_stride >= n_series)");
+  if (!(sample_period_ms > 0)) return fail(nullptr, TVBK_EINVAL, "sample period must be positive");
+  if ((flags & 4) && !start) return fail(nullptr, TVBK_EINVAL, "mean subtraction needs a run from the start");
+  if (flags & ~7) return fail(nullptr, TVBK_EINVAL, "unknown flags");
+  double par[10];
+  hipError_t e = hipMemcpy(par, params, sizeof(par), hipMemcpyDefault);
+  if (e != hipSuccess) return fail(nullptr, TVBK_EHIP, std::string("params: ") + hipGetErrorString(e));
+  e = launch_balloon(x, x_elem_bytes, x_stride, n_samples, n_series, par, flags & 1, (flags >> 1) & 1, start ? 1 : 0,
+                     (flags >> 2) & 1, sample_period_ms, state, y, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(nullptr, TVBK_EHIP, std::string("balloon: ") + hipGetErrorString(e));
+  return TVBK_OK;
+}
+
 int tvbk_monitor_layout(tvbk_plan* p, int32_t mon, int32_t* layout, int64_t* total_bytes, int64_t* offset,
                         int64_t* count, int64_t* first_step) {
   if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");

@@ -94,6 +94,10 @@ hipError_t launch_init(int precision, int model, int cf, const InitArgs& a, int 
 hipError_t launch_dense(int precision, int model, const StepArgs& a, int ntiles, int nthreads, size_t lds,
                         hipStream_t st);
 hipError_t launch_debug_slots(const uint32_t* edges, int nnz, int H, int64_t step, int32_t* out, hipStream_t st);
+// Balloon-Windkessel BOLD, one thread per series
+hipError_t launch_balloon(const void* x, int x_elem_bytes, int64_t x_stride, int64_t n_samples, int64_t n_series,
+                          const double* par, int linear, int classical, int start, int normalize,
+                          double sample_period_ms, double* state, double* y, hipStream_t st);
 // scatter a compact history [H][nch][N] (history precision) / state
 // [nsv][N] (state precision) into lane t of a tile, and the reverse gather.
 hipError_t launch_scatter(int precision, bool gather, void* ring, unsigned char* img_state, void* hist,

@@ -207,6 +207,14 @@ class Plan:
                                                 L.ptr(off), L.ptr(cnt), L.ptr(first)))
         return lay.value, tot.value, off, cnt, first
 
+    def device_output(self, m):
+        """(device pointer, bytes) of monitor m's output buffer; valid until
+        the next run or close.  Batch-lane layout [sample][voi][node][instance]."""
+        ptr = C.c_void_p()
+        nb = C.c_int64()
+        self._chk(self._lib.tvbk_monitor_device_ptr(self.handle, m, C.byref(ptr), C.byref(nb)))
+        return ptr.value, nb.value
+
     def read(self, m, instances=None):
         """Samples of monitor m from the last run: list over instances of
         (times, data (count, nvoi, N))."""

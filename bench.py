@@ -263,6 +263,31 @@ def run_c2_zero_delay(B=8192, n_steps=40000, chunk=10000, stream=None, warmup=10
     return out
 
 
+def run_balloon_on_c2(plan_chunk_steps=20000, B=8192):
+    """Balloon-Windkessel BOLD (FP64, on the device) of one C2 chunk's
+    TemporalAverage output: 1000 samples x 68 x 8192 series, timed with HIP
+    events around the kernel."""
+    import torch
+    from tvb_hip import _lib as L
+    from tvb_hip.analyzers import BalloonModel, run_balloon
+    T, n = plan_chunk_steps // 20, 68 * B
+    x = torch.rand((T, 2, n), dtype=torch.float64, device="cuda")
+    y = torch.empty((T, n), dtype=torch.float64, device="cuda")
+    st = torch.empty((4, n), dtype=torch.float64, device="cuda")
+    par = BalloonModel().params()
+    stream = torch.cuda.current_stream().cuda_stream
+    run_balloon(x.data_ptr(), 8, 2 * n, T, n, 1.0, par, 0, True, st.data_ptr(), y.data_ptr(), stream)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    run_balloon(x.data_ptr(), 8, 2 * n, T, n, 1.0, par, 0, False, st.data_ptr(), y.data_ptr(), stream)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1])
+    return {"samples": T, "series": n, "ms": ms, "series_samples_per_s": T * n / (ms * 1e-3),
+            "share_of_chunk": None}
+
+
 def run_copy(nbytes=2 << 30, reps=10):
     """Plain device copy of a known size (HBM read + write of nbytes each),
     the calibration for FETCH_SIZE / WRITE_SIZE and the measured copy
@@ -416,6 +441,9 @@ def main():
         line["p99_latency_s"] = line["serving"]["p99_latency_s"]
         line["uneven_batch"] = run_c3(stream)
         line["zero_delay_mfma"] = run_c2_zero_delay(B, stream=stream, precision=prec)
+        bal = run_balloon_on_c2(chunk, B)
+        bal["share_of_chunk"] = bal["ms"] / avg_ms
+        line["balloon_bold_fp64"] = bal
         cpu_rate, cpu_el = cpu_baseline(w, Lmat, 1.0, 1e-3, args.cpu_steps)
         line["cpu_baseline"] = {"value": cpu_rate, "unit": "region-steps/s", "cores": 1, "kind": "port",
                                 "sample": "1 instance x %d steps (%.1f s simulated) of the c2 workload, "

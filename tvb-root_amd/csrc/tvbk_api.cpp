@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -132,6 +133,8 @@ struct tvbk_plan {
   bool csr_in_lds = false;
   bool uniform_rows = false;   // batch-lane, n_channel * T a multiple of 32
   size_t srec_w_off = 0, srec_rp_off = 0;   // weights' and row starts' word offsets in d_srec
+  uint64_t par_umask = 0;                     // derived parameters shared by every instance
+  double par_uval[TVBK_KMAX_PAR] = {0};
   int mon_lds_off[TVBK_MAX_MON] = {0};
   int mon_Q[TVBK_MAX_MON] = {0};
   // device buffers
@@ -549,6 +552,16 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
       for (int v = 0; v < p->mi.nsv; ++v)
         o[p->mi.ndp + p->ncp + v] = p->stochastic ? std::sqrt(2.0 * raw[p->mi.nmp + p->ncp + v]) : 0.0;
     }
+  // derived parameters equal for every instance (and node): plan constants
+  p->par_umask = 0;
+  for (int k = 0; k < p->prow && k < TVBK_KMAX_PAR; ++k) {
+    bool same = true;
+    for (size_t r = 1; r < (size_t)B * Nsp && same; ++r) same = prm[r * p->prow + k] == prm[k];
+    if (same) {
+      p->par_umask |= 1ull << k;
+      p->par_uval[k] = prm[k];
+    }
+  }
 
   // ---- device footprint (dry run stops here)
   {
@@ -966,6 +979,7 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
   a.swts = p->d_srec.p ? (const void*)((const uint32_t*)p->d_srec.p + p->srec_w_off) : nullptr;
   a.srp = p->d_srec.p ? (const int32_t*)((const uint32_t*)p->d_srec.p + p->srec_rp_off) : nullptr;
   a.uniform_rows = p->uniform_rows ? 1 : 0;
+
   int zero_delay = 0;
   for (const ConnDesc& cd : p->conns) zero_delay |= (cd.d_min == 0 && cd.nnz > 0);
   a.zero_delay = zero_delay;
@@ -977,6 +991,8 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
   a.lds_state = p->lds_state;
   a.lds_c = p->lds_c;
   a.lds_par = p->lds_par;
+  a.par_umask = p->par_umask;
+  for (int k = 0; k < TVBK_KMAX_PAR; ++k) a.par_uval[k] = p->par_uval[k];
   a.lds_edge = p->lds_edge;
   a.lds_w = p->lds_w;
   a.lds_rp = p->lds_rp;

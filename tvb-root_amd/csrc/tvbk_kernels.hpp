@@ -5,6 +5,7 @@
 
 #define TVBK_KMAX_MON 4
 #define TVBK_KMAX_OBS 8
+#define TVBK_KMAX_PAR 32
 
 namespace tvbk {
 
@@ -66,6 +67,10 @@ struct StepArgs {
   double dt, sqrt_dt;
   int32_t lds_bold, lds_state, lds_c, lds_edge, lds_w, lds_rp;
   int32_t lds_par;   // per-lane parameters [prow][T] (state precision)
+  // parameters equal for every instance and node: bit k set -> parameter k
+  // is par_uval[k] (derived values, float64)
+  uint64_t par_umask;
+  double par_uval[TVBK_KMAX_PAR];
   // dense zero-delay kernel: W padded to [16*RB][KP] (float), KB = KP/4,
   // LDS offset of the double-buffered coupling-channel array X
   const float* dense_w;

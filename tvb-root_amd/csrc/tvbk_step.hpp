@@ -120,6 +120,12 @@ __device__ inline void step_info(const MonArgs& mo, int k, StepInfo& si) {
   }
 }
 
+// Parameter k of this lane from `par` (stride pst: registers or LDS).
+template <class R>
+__device__ __forceinline__ R lane_param(const StepArgs& a, const R* par, int pst, int k) {
+  return par[k * pst];
+}
+
 // Per-lane parameter rows (model, coupling, noise gfun; node 0's row for
 // spatial parameters) staged in LDS as [param][lane] with stride T, read by
 // phase B instead of being held in registers across the step loop.
@@ -138,7 +144,7 @@ __device__ inline void stage_params(const StepArgs& a, unsigned char* smem, int 
 // Integrator step, history write and monitors of one (node i, lane t) pair
 // with its coupling terms c (phase B of the step kernels).  `sink(ch, v)`
 // stores history channel ch of the new state.
-template <class R, class M, int CF, class NR, class SINK>
+template <class R, class M, int CF, class NR, bool UM = false, class SINK>
 __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, unsigned char* smem, R* s_x, int i,
                                             int t, int T, int N, bool live, int inst, uint64_t stream, int k,
                                             int64_t s, int buf, const R* par, int pst, const R* c,
@@ -155,12 +161,14 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
   // this lane's parameters (registers, stride 1, or LDS, stride T): model,
   // coupling, noise gfun
   R pp[NDP > 0 ? NDP : 1], cp[NCP > 0 ? NCP : 1], gf[NSV];
+  // model parameters: plan constants from the kernel arguments (UM), else
+  // this lane's
 #pragma unroll
-  for (int v = 0; v < NDP; ++v) pp[v] = par[v * pst];
+  for (int v = 0; v < NDP; ++v) pp[v] = UM ? (R)a.par_uval[v] : lane_param<R>(a, par, pst, v);
 #pragma unroll
-  for (int v = 0; v < NCP; ++v) cp[v] = par[(NDP + v) * pst];
+  for (int v = 0; v < NCP; ++v) cp[v] = lane_param<R>(a, par, pst, NDP + v);
 #pragma unroll
-  for (int v = 0; v < NSV; ++v) gf[v] = (a.noise != NOISE_NONE) ? par[(NDP + NCP + v) * pst] : R(0);
+  for (int v = 0; v < NSV; ++v) gf[v] = (a.noise != NOISE_NONE) ? lane_param<R>(a, par, pst, NDP + NCP + v) : R(0);
   if (a.spatial) {
     const R* pr = reinterpret_cast<const R*>(a.params) + (int64_t)inst * spatial_row + (int64_t)i * a.prow;
 #pragma unroll
@@ -312,9 +320,10 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
 // and twice the registers.
 // PIPE = 1: the pipelined loop only (wave-uniform rows, every delay >= 1,
 // n_channel * T <= 64); PIPE = 0: the sequential loop with every gather
-// variant.  Separate instantiations keep each loop's register allocation
-// to itself.
-template <class RH, class R, class M, int CF, int PIPE>
+// variant.  Separate instantiations keep each loop's register allocation to
+// itself.  UM: every model parameter is a plan constant (a coupling or noise
+// sweep) and is read from the kernel arguments.
+template <class RH, class R, class M, int CF, int PIPE, bool UM>
 __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs a) {
   using C = Cfun<M, CF>;
   constexpr int NSV = M::NSV, NCH = C::NCH, NCT = C::NCT, NDP = M::NDP, NCP = C::NCP;
@@ -609,10 +618,13 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   // one (node i, lane t) update of phase B for step s = s0 + k
   auto b_item = [&](int i, const RH* cbuf, int k, int64_t s_, int buf, int wslot) {
     R c[NCT];
+    R cpv[NCP > 0 ? NCP : 1];
 #pragma unroll
-    for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(i * NCT + v) * T + t], par + NDP * pst, pst);
+    for (int v = 0; v < NCP; ++v) cpv[v] = lane_param<R>(a, par, pst, NDP + v);
+#pragma unroll
+    for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(i * NCT + v) * T + t], cpv);
     RH* wr = ring + (wslot * N + i) * rstride + t;
-    update_node<R, M, CF, RH>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s_, buf, par, pst, c, bad,
+    update_node<R, M, CF, RH, UM>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s_, buf, par, pst, c, bad,
                               [&](int ch, R v) { wr[ch * T] = (RH)v; });
   };
 
@@ -621,7 +633,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   // previous barrier, so each wave interleaves its rows of phase A(s+1) with
   // its updates of phase B(s) -- a row's loads are in flight while the wave
   // integrates -- and one barrier per step orders both against the next.
-  if constexpr (PIPE) {
+  if constexpr (PIPE == 1) {
     phase_a_uni(s_c);                                   // coupling of the first step
     __syncthreads();
     // rows from the last wave down: the first waves carry the extra phase B
@@ -791,8 +803,11 @@ __global__ void __launch_bounds__(1024) k_dense(StepArgs a) {
       const int i = r0 + v;
       if (i < N) {
         R c[NCH];
+        R cpv[NCP > 0 ? NCP : 1];
 #pragma unroll
-        for (int ch = 0; ch < NCH; ++ch) c[ch] = C::post((R)d[ch][v], par + NDP * pst, pst);
+        for (int u = 0; u < NCP; ++u) cpv[u] = lane_param<R>(a, par, pst, NDP + u);
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) c[ch] = C::post((R)d[ch][v], cpv);
         update_node<R, M, CF, RH>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s, buf, par, pst, c, bad,
                                   [&](int ch, R val) { Xn[((int64_t)ch * KP + i) * T + t] = (RH)val; });
       }
@@ -858,7 +873,9 @@ __global__ void k_init_uniform(InitArgs a) {
 template <class RH, class R, class M, int CF>
 hipError_t launch_step_t(const StepArgs& a, int ntiles, int nthreads, size_t lds, hipStream_t st) {
   const bool pipe = a.uniform_rows && !a.zero_delay && Cfun<M, CF>::NCH * a.T <= 64;
-  auto kern = pipe ? k_step<RH, R, M, CF, 1> : k_step<RH, R, M, CF, 0>;
+  const uint64_t mmask = M::NDP >= 64 ? ~0ull : ((1ull << M::NDP) - 1);
+  const bool um = pipe && (a.par_umask & mmask) == mmask && !a.spatial;
+  auto kern = !pipe ? k_step<RH, R, M, CF, 0, false> : um ? k_step<RH, R, M, CF, 1, true> : k_step<RH, R, M, CF, 1, false>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;

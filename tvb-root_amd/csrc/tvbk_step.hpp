@@ -616,7 +616,10 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   };
 
   // one (node i, lane t) update of phase B for step s = s0 + k
-  auto b_item = [&](int i, const RH* cbuf, int k, int64_t s_, int buf, int wslot) {
+  // With `stage`, the new history channels go to this item's entries of the
+  // coupling buffer it has just read (NCT == NCH) instead of the ring, and
+  // ring_out() stores them later.
+  auto b_item = [&](int i, RH* cbuf, int k, int64_t s_, int buf, int wslot, bool stage) {
     R c[NCT];
     R cpv[NCP > 0 ? NCP : 1];
 #pragma unroll
@@ -624,8 +627,20 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
 #pragma unroll
     for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(i * NCT + v) * T + t], cpv);
     RH* wr = ring + (wslot * N + i) * rstride + t;
+    RH* st = cbuf + (i * NCT) * T + t;
     update_node<R, M, CF, RH, UM>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s_, buf, par, pst, c, bad,
-                              [&](int ch, R v) { wr[ch * T] = (RH)v; });
+                                  [&](int ch, R v) {
+                                    if (stage) st[ch * T] = (RH)v;
+                                    else wr[ch * T] = (RH)v;
+                                  });
+  };
+  // store this thread's staged history channels of the step into slot wslot
+  auto ring_out = [&](const RH* cbuf, int wslot) {
+    for (int i = tid / T; i < N; i += nthr / T) {
+      RH* wr = ring + (wslot * N + i) * rstride + t;
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) wr[ch * T] = cbuf[(i * NCT + ch) * T + t];
+    }
   };
 
   // ---- pipelined loop (uniform rows, one wave pass per node, every delay
@@ -645,10 +660,13 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     const uint32_t lane_off = lane < QN ? (uint32_t)lane * (uint32_t)sizeof(RH) : 0u;
     for (int k = 0; k < nsteps; ++k, ++s) {
       const int buf = k & 1;
-      const RH* cbuf = s_c + (int64_t)buf * N * NCT * T;
+      RH* cbuf = s_c + (int64_t)buf * N * NCT * T;
       RH* cnext = s_c + (int64_t)(buf ^ 1) * N * NCT * T;
       const int wslot = (smod + 1 == H) ? 0 : smod + 1;   // s % H
       const uint32_t base_n = __builtin_amdgcn_readfirstlane((uint32_t)wslot * sstride);   // (s+1-1) % H
+      // history stores wait until the step's gathers are reduced: a wave's
+      // memory counter retires in issue order, so a store issued between a
+      // row's loads and their use would delay the use until it completes
       // rows u and u+1 in flight: row u+1 is issued, then this wave's
       // update u runs, then row u is reduced; va / vb alternate
       RH va[RP], vb[RP];
@@ -657,13 +675,14 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
       auto body = [&](int u, RH* vf, int ef, int nf, RH* vi, int& ei, int& ni) {
         if (u + 1 < nrow) ni = row_issue(RPc{}, w0 + (u + 1) * nwave, base_n, lane_off, vi, ei);
         const int ib = tid / T + u * (nthr / T);
-        if (ib < N) b_item(ib, cbuf, k, s, buf, wslot);
+        if (ib < N) b_item(ib, cbuf, k, s, buf, wslot, NCT == NCH);
         if (u < nrow) row_store(cnext, w0 + u * nwave, lane, row_finish(RPc{}, ef, nf, base_n, lane_off, vf));
       };
       for (int u = 0; u < nu; u += 2) {
         body(u, va, ea, na, vb, eb, nb);
         if (u + 1 < nu) body(u + 1, vb, eb, nb, va, ea, na);
       }
+      if (NCT == NCH) ring_out(cbuf, wslot);
       if (tid == 0)
         for (int m = 0; m < a.n_mon; ++m) step_info(kc.mon[m], k + 1, kc.si[buf ^ 1][m]);
       smod = wslot;
@@ -680,7 +699,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     __syncthreads();
     // ================= phase B: integrate, write history, monitors =========
     const int wslot = (smod + 1 == H) ? 0 : smod + 1;   // s % H
-    for (int i = tid / T; i < N; i += nthr / T) b_item(i, cbuf, k, s, buf, wslot);
+    for (int i = tid / T; i < N; i += nthr / T) b_item(i, cbuf, k, s, buf, wslot, false);
     // bookkeeping of step k+1 into the other buffer (read after the next
     // phase A barrier; its previous readers finished before this step's)
     if (tid == 0)

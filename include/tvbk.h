@@ -126,6 +126,11 @@ typedef struct {
   const double *taps;       /* [S] HRF G(t_k), forward time order */
   double k1_v0;             /* FirstOrderVolterra k_1 * V_0 */
   int32_t subtract_one;     /* 1: (dot - 1) * k1_v0; 0: dot */
+  int32_t keep_stock;       /* 1: also keep every interim mean in a device ring
+                               shaped like the reference's `_stock`
+                               (monitors.py:927-931, S * n_voi * n_node * 8
+                               bytes per instance), which tvbk_export_monitor
+                               hands out for checkpoints */
 } tvbk_monitor;
 
 typedef struct {
@@ -240,6 +245,36 @@ int tvbk_monitor_device_ptr(tvbk_plan *plan, int32_t mon, void **ptr, int64_t *b
  * tvbk_set_history layout, state as float64 and current step of `inst`. */
 int tvbk_export_state(tvbk_plan *plan, int32_t inst, void *history,
                       int32_t elem_bytes, double *state, int64_t *current_step);
+
+/* Monitor state of instance `inst` in the layout of the reference's monitor
+ * stocks, the arrays SimulationHistory stores per monitor
+ * (tvb_framework/tvb/core/entities/file/simulator/simulation_history_h5.py:
+ * 83-88; monitors.py:372-391, :927-948), float64, host or device pointers:
+ *   TAVG  stock [istep][n_voi][n_node] (TemporalAverage._stock); `interim`
+ *         unused.
+ *   BOLD  stock [S][n_voi][n_node] (Bold._stock: interim mean k in slot
+ *         (k - 1) % S, needs keep_stock) and interim
+ *         [interim_istep][n_voi][n_node] (Bold._interim_stock); either may
+ *         be NULL.
+ *   RAW   no state; nothing is written.
+ * The device keeps the running sum of the current averaging window, not its
+ * samples: export writes that sum to slot 0 and zeros to the other slots,
+ * which yields the same mean, bit for bit, under the reference's sequential
+ * numpy.mean over axis 0 (float32 plans: the Kahan sum minus its
+ * compensation). */
+int tvbk_export_monitor(tvbk_plan *plan, int32_t inst, int32_t mon, double *stock, double *interim);
+
+/* Inverse of tvbk_export_monitor at the instance's current step (set it first
+ * with tvbk_set_history): TAVG and the Bold interim take the sum, in slot
+ * order, of the slots the current window has filled; BOLD rebuilds the
+ * pending HRF sums of every future emission from the stock exactly as the
+ * reference's roll-and-dot would read it (and the stock ring itself with
+ * keep_stock).  interim NULL = zeros, the state SimulationHistory.fill_into
+ * leaves (it restores `_stock` only, simulation_history_h5.py:102-104).
+ * inst = -1 applies the same arrays to every instance (one shared
+ * connectome). */
+int tvbk_import_monitor(tvbk_plan *plan, int32_t inst, int32_t mon, const double *stock,
+                        const double *interim);
 
 /* Per-instance status flags after the last run: bit 0 = a non-finite state
  * value was produced.  flags[n_inst]. */

@@ -81,3 +81,33 @@ def per_region_corr(a, b):
             continue
         cs.append(np.corrcoef(a[:, k], b[:, k])[0, 1])
     return float(min(cs)) if cs else 1.0
+
+
+def oracle_record(o):
+    """The reference's SimulationHistory.populate_from applied to an oracle
+    simulator (simulation_history_h5.py:76-88): ring as the (H, ncvar, N, 1)
+    buffer, current state and step, each monitor's ``_stock``."""
+    from tvb_hip.checkpoint import SimulationHistory
+    h = SimulationHistory(history=o.ring[..., np.newaxis].copy(), current_state=o.current_state.copy(),
+                          current_step=o.current_step)
+    h.monitor_names = [m.kind for m in o.monitors]
+    for i, m in enumerate(o.monitors):
+        setattr(h, "monitor_stock_%d" % (i + 1), None if m.kind == "Raw" else m._stock.copy())
+    return h
+
+
+def oracle_fill(o, h, interim=None):
+    """SimulationHistory.fill_into applied to a freshly built oracle
+    simulator (simulation_history_h5.py:98-107): the interim windows of Bold
+    monitors stay as configured (zeros) unless ``interim`` maps a monitor
+    index to one."""
+    o.ring[...] = np.asarray(h.history)[..., 0]
+    o.current_step = int(h.current_step)
+    o.current_state = np.array(h.current_state, copy=True)
+    for i, m in enumerate(o.monitors):
+        st = getattr(h, "monitor_stock_%d" % (i + 1))
+        if st is not None:
+            m._stock = np.array(st, copy=True)
+        if interim and i in interim:
+            m._interim_stock = np.array(interim[i], copy=True)
+    return o

@@ -140,6 +140,7 @@ struct tvbk_plan {
   // device buffers
   DevBuf d_tiles, d_conns, d_edges, d_w, d_rp, d_ring, d_img, d_bacc, d_params, d_flags, d_streams, d_z, d_srec;
   DevBuf d_taps[TVBK_MAX_MON], d_out[TVBK_MAX_MON];
+  DevBuf d_bstock[TVBK_MAX_MON];   // Bold monitors with keep_stock: interim-mean rings
   DevBuf d_dense;             // dense zero-delay kernel: W [16*RB][KP] float
   int dense_kb = 0, dense_rb = 0, lds_x = 0;
   int64_t z_steps = 0;
@@ -228,6 +229,7 @@ static void destroy(tvbk_plan* p) {
   for (int m = 0; m < TVBK_MAX_MON; ++m) {
     dev_free(&p->d_taps[m]);
     dev_free(&p->d_out[m]);
+    dev_free(&p->d_bstock[m]);
   }
   dev_free(&p->d_dense);
   if (p->ev0) hipEventDestroy(p->ev0);
@@ -500,6 +502,7 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
     });
   }
   int64_t ring_off = 0, img_off = 0, bacc_off = 0;
+  int64_t bst_len[TVBK_MAX_MON] = {0};   // Bold stock rings (doubles)
   const int ntiles = (B + T - 1) / T;
   p->tiles.resize(ntiles);
   for (int k = 0; k < ntiles; ++k) {
@@ -530,6 +533,10 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
       if (d.mon[m].kind == TVBK_MON_BOLD) {
         td.bacc_off[m] = bacc_off;
         bacc_off += (int64_t)p->mon_Q[m] * d.mon[m].n_voi * cd.n_node * T;
+        if (d.mon[m].keep_stock) {
+          td.bst_off[m] = bst_len[m];
+          bst_len[m] += (int64_t)d.mon[m].n_taps * d.mon[m].n_voi * cd.n_node * T;
+        }
       }
   }
 
@@ -572,7 +579,7 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
                  (p->uniform_rows ? (size_t)(p->conns[0].nnz + 4 * p->conns[0].n_node + TVBK_ROW_PAD) * (8 + p->hsize) +
                                         (size_t)(p->conns[0].n_node + 1) * 4 + 64 : 0);
     for (int m = 0; m < d.n_mon; ++m) {
-      if (d.mon[m].kind == TVBK_MON_BOLD) tot += p->taps_host[m].size() * 8;
+      if (d.mon[m].kind == TVBK_MON_BOLD) tot += p->taps_host[m].size() * 8 + (size_t)bst_len[m] * 8;
       const int64_t cap = d.max_steps / d.mon[m].istep + 1;
       int64_t elems = 0;
       if (T > 1) elems = cap * d.mon[m].n_voi * (int64_t)p->conns[0].n_node * B;
@@ -668,8 +675,10 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
     if ((rc = up(&p->d_dense, wd.data(), wd.size() * 4))) return bail(rc);
   }
   for (int m = 0; m < d.n_mon; ++m)
-    if (d.mon[m].kind == TVBK_MON_BOLD)
+    if (d.mon[m].kind == TVBK_MON_BOLD) {
       if ((rc = up(&p->d_taps[m], p->taps_host[m].data(), p->taps_host[m].size() * 8))) return bail(rc);
+      if ((rc = dev_alloc(p, &p->d_bstock[m], (size_t)bst_len[m] * 8))) return bail(rc);
+    }
   if ((rc = dev_alloc(p, &p->d_ring, (size_t)ring_off * p->hsize))) return bail(rc);
   if ((rc = dev_alloc(p, &p->d_img, (size_t)img_off))) return bail(rc);
   if ((rc = dev_alloc(p, &p->d_bacc, (size_t)bacc_off * 8))) return bail(rc);
@@ -1020,6 +1029,7 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
     ma.sub1 = mo.subtract_one;
     ma.k1v0 = mo.k1_v0;
     ma.taps = (const double*)p->d_taps[m].p;
+    ma.stock = (double*)p->d_bstock[m].p;
     ma.lds_off = p->mon_lds_off[m];
   }
   a.dense_w = (const float*)p->d_dense.p;
@@ -1112,6 +1122,203 @@ int tvbk_monitor_device_ptr(tvbk_plan* p, int32_t mon, void** ptr, int64_t* byte
   if (mon < 0 || mon >= p->d.n_mon) return fail(p, TVBK_EINVAL, "monitor out of range");
   if (ptr) *ptr = p->d_out[mon].p;
   if (bytes) *bytes = p->mon_bytes[mon];
+  return TVBK_OK;
+}
+
+// ---- monitor state in the reference's stock layout (checkpoints)
+
+namespace {
+
+// Host copy of one tile's state for monitor `m`: its image (TemporalAverage
+// sums, Bold interim sums), Bold HRF accumulators and stock ring.
+struct MonTile {
+  std::vector<unsigned char> img;
+  std::vector<double> acc, stk;
+};
+
+int mon_tile_io(tvbk_plan* p, int m, const TileDesc& td, MonTile& mt, bool get, bool with_stock) {
+  const tvbk_monitor& mo = p->d.mon[m];
+  const int N = p->conns[td.conn].n_node;
+  unsigned char* dimg = (unsigned char*)p->d_img.p + td.img_off;
+  const hipMemcpyKind k = get ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice;
+  mt.img.resize(p->img_bytes);
+  HIPCHK(p, get ? hipMemcpy(mt.img.data(), dimg, mt.img.size(), k) : hipMemcpy(dimg, mt.img.data(), mt.img.size(), k));
+  if (mo.kind == TVBK_MON_BOLD) {
+    double* dacc = (double*)p->d_bacc.p + td.bacc_off[m];
+    mt.acc.resize((size_t)p->mon_Q[m] * mo.n_voi * N * p->T);
+    const size_t ab = mt.acc.size() * 8;
+    HIPCHK(p, get ? hipMemcpy(mt.acc.data(), dacc, ab, k) : hipMemcpy(dacc, mt.acc.data(), ab, k));
+    if (with_stock && mo.keep_stock) {
+      double* dst = (double*)p->d_bstock[m].p + td.bst_off[m];
+      mt.stk.resize((size_t)mo.n_taps * mo.n_voi * N * p->T);
+      const size_t sb = mt.stk.size() * 8;
+      HIPCHK(p, get ? hipMemcpy(mt.stk.data(), dst, sb, k) : hipMemcpy(dst, mt.stk.data(), sb, k));
+    }
+  }
+  return TVBK_OK;
+}
+
+int mon_args_ok(tvbk_plan* p, int32_t inst, int32_t mon, bool allow_all) {
+  if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
+  if (inst < (allow_all ? -1 : 0) || inst >= p->d.n_inst) return fail(p, TVBK_EINVAL, "instance out of range");
+  if (mon < 0 || mon >= p->d.n_mon) return fail(p, TVBK_EINVAL, "monitor out of range");
+  return TVBK_OK;
+}
+
+}  // namespace
+
+int tvbk_export_monitor(tvbk_plan* p, int32_t inst, int32_t mon, double* stock, double* interim) {
+  int rc = mon_args_ok(p, inst, mon, false);
+  if (rc) return rc;
+  const tvbk_monitor& mo = p->d.mon[mon];
+  if (mo.kind == TVBK_MON_RAW) return TVBK_OK;
+  if (mo.kind == TVBK_MON_BOLD && stock && !mo.keep_stock)
+    return fail(p, TVBK_ESTATE, "the Bold stock is only kept by a monitor created with keep_stock");
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  const TileDesc& td = p->tiles[p->tile_of[inst]];
+  const int N = p->conns[td.conn].n_node, nv = mo.n_voi, T = p->T, t = p->lane_of[inst];
+  MonTile mt;
+  if ((rc = mon_tile_io(p, mon, td, mt, true, stock != nullptr))) return rc;
+  const unsigned char* reg = mt.img.data() + p->mon_lds_off[mon];
+  if (mo.kind == TVBK_MON_TAVG) {
+    if (!stock) return TVBK_OK;
+    std::vector<double> out((size_t)mo.istep * nv * N, 0.0);
+    for (int i = 0; i < N; ++i)
+      for (int v = 0; v < nv; ++v) {
+        double sum;
+        if (p->esize == 8) {
+          sum = reinterpret_cast<const double*>(reg)[(i * nv + v) * T + t];
+        } else {   // Kahan pair (sum, compensation): the true sum is sum - compensation
+          const float* f = reinterpret_cast<const float*>(reg);
+          sum = (double)f[((i * nv + v) * 2 + 0) * T + t] - (double)f[((i * nv + v) * 2 + 1) * T + t];
+        }
+        out[(size_t)v * N + i] = sum;
+      }
+    HIPCHK(p, hipMemcpy(stock, out.data(), out.size() * 8, hipMemcpyDefault));
+    return TVBK_OK;
+  }
+  // Bold
+  if (interim) {
+    std::vector<double> out((size_t)mo.interim_istep * nv * N, 0.0);
+    const double* bi = reinterpret_cast<const double*>(reg);
+    for (int i = 0; i < N; ++i)
+      for (int v = 0; v < nv; ++v) out[(size_t)v * N + i] = bi[(i * nv + v) * T + t];
+    HIPCHK(p, hipMemcpy(interim, out.data(), out.size() * 8, hipMemcpyDefault));
+  }
+  if (stock) {
+    const int S = mo.n_taps;
+    std::vector<double> out((size_t)S * nv * N);
+    for (int sl = 0; sl < S; ++sl)
+      for (int v = 0; v < nv; ++v)
+        for (int i = 0; i < N; ++i)
+          out[((size_t)sl * nv + v) * N + i] = mt.stk[(((size_t)sl * nv + v) * N + i) * T + t];
+    HIPCHK(p, hipMemcpy(stock, out.data(), out.size() * 8, hipMemcpyDefault));
+  }
+  return TVBK_OK;
+}
+
+int tvbk_import_monitor(tvbk_plan* p, int32_t inst, int32_t mon, const double* stock, const double* interim) {
+  int rc = mon_args_ok(p, inst, mon, true);
+  if (rc) return rc;
+  const tvbk_monitor& mo = p->d.mon[mon];
+  if (mo.kind == TVBK_MON_RAW) return TVBK_OK;
+  if (!stock) return fail(p, TVBK_EINVAL, "null stock");
+  const int B = p->d.n_inst, T = p->T;
+  const int b0 = inst < 0 ? 0 : inst, b1 = inst < 0 ? B : inst + 1;
+  const int conn = conn_of_inst(p, b0);
+  for (int b = b0; b < b1; ++b)
+    if (conn_of_inst(p, b) != conn) return fail(p, TVBK_EINVAL, "inst=-1 needs one shared connectome");
+  const int N = p->conns[conn].n_node, nv = mo.n_voi;
+  const bool bold = mo.kind == TVBK_MON_BOLD;
+  const int S = mo.n_taps, ii = bold ? mo.interim_istep : 1;
+  std::vector<double> hs((size_t)(bold ? S : mo.istep) * nv * N), hi;
+  HIPCHK(p, hipMemcpy(hs.data(), stock, hs.size() * 8, hipMemcpyDefault));
+  if (bold && interim) {
+    hi.resize((size_t)ii * nv * N);
+    HIPCHK(p, hipMemcpy(hi.data(), interim, hi.size() * 8, hipMemcpyDefault));
+  }
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  const std::vector<double>& taps = p->taps_host[mon];
+  std::vector<double> pend;       // Bold: pending HRF sums [Q][n_voi][n_node] at step pend_step
+  int64_t pend_step = -1;
+  // tiles holding the instances, each read, changed lane by lane and written back once
+  std::vector<int> tiles;
+  for (int b = b0; b < b1; ++b)
+    if (tiles.empty() || tiles.back() != p->tile_of[b]) tiles.push_back(p->tile_of[b]);
+  std::sort(tiles.begin(), tiles.end());
+  tiles.erase(std::unique(tiles.begin(), tiles.end()), tiles.end());
+  for (int k : tiles) {
+    const TileDesc& td = p->tiles[k];
+    MonTile mt;
+    if ((rc = mon_tile_io(p, mon, td, mt, true, true))) return rc;
+    unsigned char* reg = mt.img.data() + p->mon_lds_off[mon];
+    for (int l = 0; l < td.n_inst; ++l) {
+      const int b = td.inst0 + l;
+      if (b < b0 || b >= b1) continue;
+      const int t = p->lane_of[b];
+      const int64_t s = p->cur_step[b];
+      if (!bold) {
+        // TemporalAverage: samples of steps s - kf + 1 .. s sit in slots 0 .. kf - 1 (monitors.py:386)
+        const int kf = (int)(s % mo.istep);
+        for (int i = 0; i < N; ++i)
+          for (int v = 0; v < nv; ++v) {
+            double sum = 0.0;
+            for (int j = 0; j < kf; ++j) sum = sum + hs[((size_t)j * nv + v) * N + i];
+            if (p->esize == 8) {
+              reinterpret_cast<double*>(reg)[(i * nv + v) * T + t] = sum;
+            } else {
+              float* f = reinterpret_cast<float*>(reg);
+              const float sf = (float)sum;
+              f[((i * nv + v) * 2 + 0) * T + t] = sf;
+              f[((i * nv + v) * 2 + 1) * T + t] = (float)((double)sf - sum);
+            }
+          }
+        continue;
+      }
+      // Bold interim window (monitors.py:944)
+      const int kf = (int)(s % ii);
+      double* bi = reinterpret_cast<double*>(reg);
+      for (int i = 0; i < N; ++i)
+        for (int v = 0; v < nv; ++v) {
+          double sum = 0.0;
+          if (!hi.empty())
+            for (int j = 0; j < kf; ++j) sum = sum + hi[((size_t)j * nv + v) * N + i];
+          bi[(i * nv + v) * T + t] = sum;
+        }
+      // pending emissions kk (kk * istep > s): the HRF sum over the interim
+      // samples mp <= s / ii already taken, in the order the step kernel adds
+      // them (ascending mp); sample mp sits in stock slot (mp - 1) mod S and
+      // enters emission kk with age floor(kk * istep / ii) - mp in [0, S)
+      // (monitors.py:948-960)
+      const int Q = p->mon_Q[mon];
+      if (pend_step != s) {   // every instance reads the same stock: one evaluation per distinct step
+        pend_step = s;
+        pend.assign((size_t)Q * nv * N, 0.0);
+        const int64_t mpc = s / ii, kk0 = s / mo.istep + 1;
+        for (int64_t kk = kk0; kk < kk0 + Q; ++kk) {
+          const int64_t me = kk * mo.istep / ii;
+          const int64_t lo = me - S + 1, hi_mp = std::min(me, mpc);
+          double* dst = &pend[(size_t)(kk % Q) * nv * N];
+          for (int64_t mp = lo; mp <= hi_mp; ++mp) {
+            const int64_t age = me - mp;
+            const double w = taps[age == 0 ? S - 1 : age - 1];
+            const double* src = &hs[(size_t)(((mp - 1) % S + S) % S) * nv * N];
+            for (int q = 0; q < nv * N; ++q) dst[q] = dst[q] + w * src[q];
+          }
+        }
+      }
+      for (int q = 0; q < Q; ++q)
+        for (int v = 0; v < nv; ++v)
+          for (int i = 0; i < N; ++i)
+            mt.acc[(((size_t)q * nv + v) * N + i) * T + t] = pend[((size_t)q * nv + v) * N + i];
+      if (mo.keep_stock)
+        for (int sl = 0; sl < S; ++sl)
+          for (int v = 0; v < nv; ++v)
+            for (int i = 0; i < N; ++i)
+              mt.stk[(((size_t)sl * nv + v) * N + i) * T + t] = hs[((size_t)sl * nv + v) * N + i];
+    }
+    if ((rc = mon_tile_io(p, mon, td, mt, false, true))) return rc;
+  }
   return TVBK_OK;
 }
 

@@ -25,6 +25,7 @@ struct TileDesc {
   int64_t step0, n_steps;          // first step of this call, steps to run
   int64_t ring_off, img_off;       // ring elements, image bytes
   int64_t bacc_off[TVBK_KMAX_MON]; // Bold accumulators (doubles)
+  int64_t bst_off[TVBK_KMAX_MON];  // Bold stock ring (doubles), monitors keeping one
   int64_t out_off[TVBK_KMAX_MON], out_sj[TVBK_KMAX_MON], out_sv[TVBK_KMAX_MON], out_si[TVBK_KMAX_MON];
   int64_t z_off, z_sk, z_ssv, z_si;  // injected noise addressing
 };
@@ -35,6 +36,7 @@ struct MonArgs {
   int32_t voi[TVBK_KMAX_OBS];
   double k1v0;
   const double* taps;
+  double* stock;      // Bold: ring of interim means [S][nvoi][N][T] per tile, or null
   // per-tile phase of the first step s0 of a call, filled on the device:
   // r0 = s0 % istep, qe0 = s0 / istep, rb0 = s0 % ii, qb0 = s0 / ii
   int32_t r0, rb0;

@@ -274,6 +274,10 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
         if (push) {
           const double mean = *bsum / (double)mo.ii;
           *bsum = 0.0;
+          if (mo.stock) {   // the reference's stock slot (mp - 1) % S (monitors.py:948), mp >= 1
+            const int64_t sl = (mp - 1) % mo.S;
+            mo.stock[td.bst_off[m] + ((sl * nv + v) * (int64_t)N + i) * T + t] = mean;
+          }
           // sample mp contributes to emissions kk with kk*istep >= mp*ii
           // and kk*istep < (mp+S)*ii; weight by age a = floor(kk*istep/ii) - mp:
           // a == 0 -> G[S-1], else G[a-1]

@@ -47,7 +47,7 @@ class Monitor(C.Structure):
     _fields_ = [("kind", C.c_int32), ("istep", C.c_int32), ("n_voi", C.c_int32),
                 ("voi", C.c_int32 * MAX_OBS), ("interim_istep", C.c_int32),
                 ("n_taps", C.c_int32), ("taps", C.c_void_p), ("k1_v0", C.c_double),
-                ("subtract_one", C.c_int32)]
+                ("subtract_one", C.c_int32), ("keep_stock", C.c_int32)]
 
 
 class PlanDesc(C.Structure):
@@ -86,6 +86,8 @@ def _bind(lib):
         "tvbk_read_monitor": ([P, C.c_int32, P, C.c_int64], C.c_int),
         "tvbk_monitor_device_ptr": ([P, C.c_int32, P, P], C.c_int),
         "tvbk_export_state": ([P, C.c_int32, P, C.c_int32, P, P], C.c_int),
+        "tvbk_export_monitor": ([P, C.c_int32, C.c_int32, P, P], C.c_int),
+        "tvbk_import_monitor": ([P, C.c_int32, C.c_int32, P, P], C.c_int),
         "tvbk_read_flags": ([P, P], C.c_int),
         "tvbk_debug_slots": ([P, C.c_int32, C.c_int64, P], C.c_int),
         "tvbk_plan_shape": ([P, P, P, P, P], C.c_int),
@@ -106,7 +108,8 @@ EXPORTED = (
     "tvbk_plan_footprint", "tvbk_poll", "tvbk_stream_create", "tvbk_stream_destroy",
     "tvbk_set_history", "tvbk_init_uniform", "tvbk_set_noise", "tvbk_run_steps",
     "tvbk_set_steps", "tvbk_sync", "tvbk_monitor_layout", "tvbk_read_monitor",
-    "tvbk_monitor_device_ptr", "tvbk_export_state", "tvbk_read_flags",
+    "tvbk_monitor_device_ptr", "tvbk_export_state", "tvbk_export_monitor", "tvbk_import_monitor",
+    "tvbk_read_flags",
     "tvbk_debug_slots", "tvbk_plan_shape", "tvbk_last_run_ms", "tvbk_balloon", "tvbk_last_error")
 
 

@@ -64,7 +64,7 @@ def observables(model):
     return np.array(rows)
 
 
-def monitor_spec(mon, dt):
+def monitor_spec(mon, dt, keep_stock=False):
     kind = MONITORS[_name(mon)]
     voi = [int(v) for v in np.atleast_1d(mon.voi)]
     if kind == L.MON_BOLD:
@@ -76,7 +76,8 @@ def monitor_spec(mon, dt):
         else:
             k1v0, sub = 1.0, False
         return MonitorSpec(kind, mon.istep, voi, interim_istep=mon._interim_istep, taps=taps,
-                           k1_v0=k1v0, subtract_one=sub, period=mon.period, name="Bold")
+                           k1_v0=k1v0, subtract_one=sub, period=mon.period, name="Bold",
+                           keep_stock=keep_stock)
     return MonitorSpec(kind, mon.istep, voi, period=getattr(mon, "period", None), name=_name(mon))
 
 
@@ -146,17 +147,54 @@ class HipBackend:
         observables(sim.model)
 
     # ------------------------------------------------------------ plan build
-    def _components(self, sim):
+    def _components(self, sim, keep_stock=False):
         model_id = MODELS[_name(sim.model)]
         cfun_id, cfun_params = CFUNS[_name(sim.coupling)]
         scheme = SCHEMES[_name(sim.integrator)]
         stochastic = scheme in (L.HEUN_STO, L.EULER_STO)
         mons = sim.monitors if isinstance(sim.monitors, (list, tuple)) else [sim.monitors]
-        specs = [monitor_spec(m, sim.integrator.dt) for m in mons]
+        specs = [monitor_spec(m, sim.integrator.dt, keep_stock) for m in mons]
         return model_id, cfun_id, cfun_params, scheme, stochastic, specs
+
+    @staticmethod
+    def _monitors(sim):
+        return sim.monitors if isinstance(sim.monitors, (list, tuple)) else [sim.monitors]
+
+    def _import_monitors(self, plan, inst, sim):
+        """Carry the simulator's monitor stocks (TemporalAverage._stock,
+        Bold._stock / _interim_stock, monitors.py:372-378, :927-935) into
+        the plan at its current step, so that a run continues the averaging
+        windows and HRF sums the reference would.  All-zero stocks (a fresh
+        configure) match the plan's zeroed state and are skipped."""
+        for m, mon in enumerate(self._monitors(sim)):
+            stock = getattr(mon, "_stock", None)
+            interim = getattr(mon, "_interim_stock", None)
+            if stock is None or plan.monitors[m].kind == L.MON_RAW:
+                continue
+            if not np.any(stock) and (interim is None or not np.any(interim)):
+                continue
+            plan.import_monitor(inst, m, stock, interim)
+
+    def _export_monitors(self, plan, inst, sim):
+        """The plan's monitor state back into the simulator's monitor stocks."""
+        for m, mon in enumerate(self._monitors(sim)):
+            ms = plan.monitors[m]
+            if ms.kind == L.MON_RAW:
+                continue
+            if ms.kind == L.MON_BOLD and not ms.keep_stock:
+                continue
+            stock, interim = plan.export_monitor(inst, m)
+            mon._stock = stock
+            if interim is not None:
+                mon._interim_stock = interim
 
     def _initial(self, sim, precision, cfun_id, csr):
         """History channels (H, nch, N) and state from the configured sim."""
+        chans = getattr(sim.history, "channels", None)
+        if chans is not None:
+            # the coupling channels a previous device run left (SigmoidalJansenRit:
+            # the pre-synaptic transform, which the reference buffer cannot hold)
+            return np.ascontiguousarray(chans), np.asarray(sim.current_state, np.float64)[..., 0]
         use64 = precision == L.F64 and getattr(sim.history, "initial_history", None) is not None
         if use64:
             hist = np.asarray(sim.history.initial_history, np.float64)[..., 0]       # (H, nsv, N)
@@ -172,11 +210,12 @@ class HipBackend:
         return np.ascontiguousarray(cv), state
 
     def build_plan(self, sim, precision="auto", noise="legacy", max_steps=65536, kernel="auto",
-                   tile=0, waves=0, seed=None, stream=None):
-        """A one-instance Plan from a configured simulator, with its history
-        and state uploaded."""
+                   tile=0, waves=0, seed=None, stream=None, keep_stock=False):
+        """A one-instance Plan from a configured simulator, with its history,
+        state and monitor stocks uploaded.  keep_stock: Bold monitors also
+        keep their interim-mean stock on the device (for export)."""
         self.check_compatibility(sim)
-        model_id, cfun_id, cfun_params, scheme, stochastic, specs = self._components(sim)
+        model_id, cfun_id, cfun_params, scheme, stochastic, specs = self._components(sim, keep_stock)
         prec = precision_id(precision, cfun_id)
         N = sim.connectivity.number_of_regions
         csr = CSR(sim.connectivity.weights, sim.connectivity.idelays, sim.connectivity.horizon)
@@ -203,6 +242,7 @@ class HipBackend:
                     tile=tile, waves=waves, stream=stream)
         hist, state = self._initial(sim, prec, cfun_id, csr)
         plan.set_history(0, hist, state, sim.current_step)
+        self._import_monitors(plan, 0, sim)
         return plan
 
     # --------------------------------------------------------------- running
@@ -222,7 +262,7 @@ class HipBackend:
         chunk = max(1, min(chunk, nstep))
         philox_noise = isinstance(noise, str) and noise == "philox"
         plan = self.build_plan(sim, precision=precision, noise="philox" if philox_noise else "inject",
-                               max_steps=chunk, kernel=kernel)
+                               max_steps=chunk, kernel=kernel, keep_stock=write_back)
         try:
             N = sim.connectivity.number_of_regions
             nsv = len(sim.model.state_variables)
@@ -246,11 +286,21 @@ class HipBackend:
                 done += n
             flags = plan.flags()
             if write_back:
-                hist, state, step = plan.export(0, np.float32)
+                sjr = _name(sim.coupling) == "SigmoidalJansenRit"
+                hist, state, step = plan.export(0, np.float64 if sjr or plan.hsize == 8 else np.float32)
                 sim.current_state = state[..., np.newaxis].copy()
                 sim.current_step = step
-                if _name(sim.coupling) != "SigmoidalJansenRit":
-                    sim.history.buffer = hist[..., np.newaxis].copy()
+                if sjr:
+                    # the ring holds the coupling's pre-synaptic transform of
+                    # y1 - y2, not y1 and y2: keep it beside the (now stale)
+                    # reference buffer for the next device run
+                    sim.history.channels = hist
+                else:
+                    sim.history.buffer = hist.astype(np.float32)[..., np.newaxis]
+                    sim.history.channels = None
+                    if plan.hsize == 8:
+                        sim.history.channels = hist
+                self._export_monitors(plan, 0, sim)
             self.last_flags = flags
             self.last_plan_shape = (plan.kernel, plan.tile, plan.waves, plan.lds_bytes)
         finally:
@@ -308,6 +358,7 @@ class HipBackend:
         for b, s in enumerate(sims):
             hist, state = self._initial(s, prec, cfun_id, conns[inst_conn[b]])
             plan.set_history(b, hist, state, s.current_step)
+            self._import_monitors(plan, b, s)
         return plan
 
     def run_batch(self, sims, nstep, precision="auto", noise="inject", steps=None, kernel="auto", tile=0,

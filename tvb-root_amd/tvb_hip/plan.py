@@ -42,7 +42,7 @@ class MonitorSpec:
     """Device monitor description (``tvbk_monitor``)."""
 
     def __init__(self, kind, istep, voi, interim_istep=0, taps=None, k1_v0=0.0,
-                 subtract_one=True, period=None, name=None):
+                 subtract_one=True, period=None, name=None, keep_stock=False):
         self.kind = kind
         self.istep = int(istep)
         self.voi = [int(v) for v in voi]
@@ -52,6 +52,7 @@ class MonitorSpec:
         self.subtract_one = bool(subtract_one)
         self.period = period
         self.name = name
+        self.keep_stock = bool(keep_stock)
 
     def times(self, first_step, count, dt):
         steps = first_step + self.istep * np.arange(count)
@@ -108,6 +109,7 @@ class Plan:
                 mo.taps = L.ptr(ms.taps)
                 mo.k1_v0 = ms.k1_v0
                 mo.subtract_one = int(ms.subtract_one)
+                mo.keep_stock = int(ms.keep_stock)
         d.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
         d.inst_stream = L.ptr(self.inst_stream)
         d.max_steps = self.max_steps
@@ -257,6 +259,43 @@ class Plan:
         self._chk(self._lib.tvbk_export_state(self.handle, inst, L.ptr(hist), hist.dtype.itemsize,
                                               L.ptr(state), C.byref(step)))
         return hist, state, step.value
+
+    def _n_node(self, inst):
+        return self.conns[0 if self.inst_conn is None or inst < 0 else self.inst_conn[inst]].n_node
+
+    def export_monitor(self, inst, m, stock=True, interim=True):
+        """Monitor m's state of instance ``inst`` as the reference's stock
+        arrays (float64, trailing mode axis): TemporalAverage ``_stock``
+        (istep, nvoi, N, 1) -> (stock, None); Bold ``_stock`` (S, nvoi, N, 1)
+        (monitor built with keep_stock) and ``_interim_stock``
+        (interim_istep, nvoi, N, 1); Raw -> (None, None)."""
+        ms = self.monitors[m]
+        if ms.kind == L.MON_RAW:
+            return None, None
+        N, nv = self._n_node(inst), len(ms.voi)
+        if ms.kind == L.MON_TAVG:
+            st, it = np.empty((ms.istep, nv, N)), None
+        else:
+            st = np.empty((len(ms.taps), nv, N)) if stock else None
+            it = np.empty((ms.interim_istep, nv, N)) if interim else None
+        self._chk(self._lib.tvbk_export_monitor(self.handle, int(inst), int(m), L.ptr(st), L.ptr(it)))
+        return (None if st is None else st[..., np.newaxis]), (None if it is None else it[..., np.newaxis])
+
+    def import_monitor(self, inst, m, stock, interim=None):
+        """Inverse of export_monitor at the instance's current step (after
+        set_history); ``interim`` None = zeros, the state a reference Bold
+        monitor has after SimulationHistory.fill_into.  inst -1 = every
+        instance."""
+        ms = self.monitors[m]
+        if ms.kind == L.MON_RAW:
+            return
+        N, nv = self._n_node(inst), len(ms.voi)
+        rows = ms.istep if ms.kind == L.MON_TAVG else len(ms.taps)
+        st = np.ascontiguousarray(np.asarray(stock, np.float64).reshape((rows, nv, N)))
+        it = None
+        if interim is not None and ms.kind == L.MON_BOLD:
+            it = np.ascontiguousarray(np.asarray(interim, np.float64).reshape((ms.interim_istep, nv, N)))
+        self._chk(self._lib.tvbk_import_monitor(self.handle, int(inst), int(m), L.ptr(st), L.ptr(it)))
 
     def flags(self):
         f = np.zeros(self.n_inst, np.int32)

@@ -320,6 +320,11 @@ class Raw(Monitor):
 class TemporalAverage(Monitor):
     """monitors.py:363-391."""
 
+    def _config_time(self, sim):
+        super()._config_time(sim)
+        # the averaging window's samples (monitors.py:372-378)
+        self._stock = np.zeros((self.istep, self.voi.shape[0], sim.number_of_nodes, 1))
+
 
 class Bold(Monitor):
     """monitors.py:834-962 (FIR of 4 ms means with the FirstOrderVolterra HRF)."""
@@ -345,6 +350,10 @@ class Bold(Monitor):
     def _config_time(self, sim):
         super()._config_time(sim)
         self.compute_hrf()
+        # interim window and interim-mean stock (monitors.py:927-935)
+        shape = (self.voi.shape[0], sim.number_of_nodes, 1)
+        self._interim_stock = np.zeros((self._interim_istep,) + shape)
+        self._stock = np.zeros((self._stock_steps,) + shape)
 
 
 # ---------------------------------------------------------------------------

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define TVBK_ABI_VERSION 1
+#define TVBK_ABI_VERSION 2
 
 enum {
   TVBK_OK = 0,
@@ -87,8 +87,23 @@ enum { TVBK_F32 = 0, TVBK_F64 = 1, TVBK_MIXED = 2 };
  *             (node | call<<20, stream(inst), step_lo, step_hi). */
 enum { TVBK_NOISE_NONE = 0, TVBK_NOISE_INJECTED = 1, TVBK_NOISE_PHILOX = 2 };
 
-/* Monitors (simulator/monitors.py:160-190, :363-391, :834-962). */
-enum { TVBK_MON_RAW = 0, TVBK_MON_TAVG = 1, TVBK_MON_BOLD = 2 };
+/* Monitors (simulator/monitors.py).  Outputs per sample: [voi][node]
+ * except GAVG ([voi][1]) and SAVG ([voi][n_area]).
+ *   RAW        Raw, every step (:160-190)
+ *   TAVG       TemporalAverage (:363-391)
+ *   BOLD       Bold with the FirstOrderVolterra-style HRF taps (:834-962)
+ *   SUBSAMPLE  SubSample, every istep steps (:205-214)
+ *   GAVG       GlobalAverage, mean over all nodes (:344-360)
+ *   SAVG       SpatialAverage, spatial_mean . state over a node -> area mask
+ *              (:217-342)
+ *   AFF        AfferentCoupling, every step: the node coupling the next step
+ *              integrates with, voi indexing the coupling variables
+ *              (:392-410; simulator.py:311-316)
+ *   AFF_TAVG   AfferentCouplingTemporalAverage (:413-431) */
+enum {
+  TVBK_MON_RAW = 0, TVBK_MON_TAVG = 1, TVBK_MON_BOLD = 2, TVBK_MON_SUBSAMPLE = 3,
+  TVBK_MON_GAVG = 4, TVBK_MON_SAVG = 5, TVBK_MON_AFF = 6, TVBK_MON_AFF_TAVG = 7
+};
 
 /* Kernel families. */
 enum {
@@ -131,6 +146,10 @@ typedef struct {
                                (monitors.py:927-931, S * n_voi * n_node * 8
                                bytes per instance), which tvbk_export_monitor
                                hands out for checkpoints */
+  /* SpatialAverage only: area index of every node (0 .. n_area - 1, each
+   * area used), plans with one connectome */
+  const int32_t *spatial_mask;
+  int32_t n_area;
 } tvbk_monitor;
 
 typedef struct {
@@ -212,6 +231,16 @@ int tvbk_init_uniform(tvbk_plan *plan, uint64_t seed, const double *lo,
  * device pointer is borrowed until the run returns).  Replaces the per-step
  * Noise.generate draws (noise.py:195-198, integrators.py:342). */
 int tvbk_set_noise(tvbk_plan *plan, const double *z, int64_t n_steps);
+
+/* Region stimulus for the next tvbk_run_steps call (simulator.py:290-306;
+ * patterns.py:116-140): the state variables in `svar_mask` (the model's
+ * stvar) receive dt * spatial[node] * temporal[k] in the integrator at the
+ * call's k-th step (integrators.py:310-315, :354-359, :388, :423, :506),
+ * spatial [n_node] and temporal [n_steps] float64 (host or device
+ * pointers; the pattern is shared by every instance).  temporal NULL
+ * removes the stimulus.  Plans with one connectome, not the dense kernel. */
+int tvbk_set_stimulus(tvbk_plan *plan, const double *spatial, const double *temporal,
+                      int64_t n_steps, uint32_t svar_mask);
 
 /* Replaces the per-step loop of Simulator.__call__ (simulator.py:345-366):
  * advances every instance by n_steps (instances with fewer remaining steps,

@@ -43,7 +43,7 @@ MODELS = {
         bounds={}),
     # simulator/models/infinite_theta.py:64-167
     "MontbrioPazoRoxin": dict(
-        svars=("r", "V"), cvar=(0, 1),
+        svars=("r", "V"), cvar=(0, 1), stvar=(1,),
         defaults=dict(tau=1.0, I=0.0, Delta=1.0, J=15.0, eta=-5.0, Gamma=0.0,
                       cr=1.0, cv=0.0),
         svr={"r": (0.0, 2.0), "V": (-2.0, 1.5)},
@@ -118,14 +118,26 @@ def first_order_volterra(t, tau_s=0.8, tau_f=0.4):
 class Monitor:
     """Stock-based monitors restated from simulator/monitors.py."""
 
-    def __init__(self, kind, period, voi, dt, n_node, hrf=None):
+    AFFERENT = ("AfferentCoupling", "AfferentCouplingTemporalAverage")
+
+    def __init__(self, kind, period, voi, dt, n_node, hrf=None, spatial_mask=None):
         self.kind = kind
         self.voi = np.asarray(voi)
         self.dt = dt
         nv = len(self.voi)
-        if kind == "Raw":  # monitors.py:178-190
+        if kind in ("Raw", "AfferentCoupling"):  # monitors.py:178-190, :392-410
             self.istep = 1
-        elif kind == "TemporalAverage":  # monitors.py:103-105, :372-378
+        elif kind in ("SubSample", "GlobalAverage"):  # monitors.py:205-214, :344-360
+            self.istep = iround(period / dt)
+        elif kind == "SpatialAverage":  # monitors.py:287-333
+            self.istep = iround(period / dt)
+            mask = np.asarray(spatial_mask, int)
+            n_area = len(np.unique(mask))
+            ss = np.zeros((n_node, n_area))
+            ss[np.arange(n_node), mask] = 1
+            ss = ss.T
+            self.spatial_mean = ss / np.sum(ss, axis=1)[:, np.newaxis]
+        elif kind in ("TemporalAverage", "AfferentCouplingTemporalAverage"):  # :103-105, :372-378, :413-431
             self.istep = iround(period / dt)
             self._stock = np.zeros((self.istep, nv, n_node, 1))
         elif kind == "Bold":  # monitors.py:903-935
@@ -150,7 +162,19 @@ class Monitor:
     def sample(self, step, observed):
         if self.kind == "Raw":  # monitors.py:188-190
             return [step * self.dt, observed]
-        if self.kind == "TemporalAverage":  # monitors.py:380-391
+        if self.kind == "AfferentCoupling":  # RawVoi.sample (monitors.py:208-211)
+            return [step * self.dt, observed[self.voi]]
+        if self.kind in ("SubSample", "GlobalAverage", "SpatialAverage"):
+            if step % self.istep != 0:
+                return None
+            x = observed[self.voi, :]
+            if self.kind == "SubSample":  # monitors.py:211-214
+                return [step * self.dt, x]
+            if self.kind == "GlobalAverage":  # monitors.py:352-357
+                return [step * self.dt, np.mean(x, axis=1)[:, np.newaxis, :]]
+            # SpatialAverage.sample (monitors.py:328-333)
+            return [step * self.dt, np.dot(self.spatial_mean, x).transpose((1, 0, 2))]
+        if self.kind in ("TemporalAverage", "AfferentCouplingTemporalAverage"):  # monitors.py:380-391
             self._stock[((step % self.istep) - 1), :] = observed[self.voi]
             if step % self.istep == 0:
                 avg = np.mean(self._stock, axis=0)
@@ -185,7 +209,7 @@ class OracleSim:
                  integrator="HeunDeterministic", nsig=1.0,
                  initial_conditions=None, noise_seed=42,
                  monitors=(("Raw", None, None),), mode="faithful",
-                 hrf=None, model_voi=None):
+                 hrf=None, model_voi=None, stimulus=None):
         assert mode in ("faithful", "fp64")
         self.mode = mode
         self.hdtype = np.float32 if mode == "faithful" else np.float64
@@ -270,12 +294,23 @@ class OracleSim:
         # variables_of_interest; Raw always records all of them (:178-181)
         self.voi_names = tuple(spec["voi"] if model_voi is None else model_voi)
         self.monitors = []
-        for kind, period, voi in monitors:
-            if kind == "Raw" or voi is None or len(voi) == 0:
+        for spec_m in monitors:
+            kind, period, voi = spec_m[:3]
+            extra = spec_m[3] if len(spec_m) > 3 else {}
+            if kind in Monitor.AFFERENT:   # indices of the coupling variables (monitors.py:405-410)
+                idx = list(range(len(self.cvar))) if voi is None or len(voi) == 0 else list(voi)
+            elif kind == "Raw" or voi is None or len(voi) == 0:
                 idx = list(range(len(self.voi_names)))
             else:
                 idx = list(voi)
-            self.monitors.append(Monitor(kind, period, idx, self.dt, N, hrf=hrf))
+            self.monitors.append(Monitor(kind, period, idx, self.dt, N, hrf=hrf, **extra))
+        # region stimulus: (spatial pattern (N,), temporal pattern (T,)) applied
+        # to the model's stvar (simulator.py:290-306; models/base.py:91-92)
+        self.stvar = list(spec.get("stvar", spec["cvar"]))
+        self.stimulus = None
+        if stimulus is not None:
+            sp, tp = stimulus
+            self.stimulus = (np.asarray(sp, float).reshape((-1, 1)), np.asarray(tp, float).reshape((1, -1)))
 
     # ---------------- components --------------------------------------------
 
@@ -364,36 +399,37 @@ class OracleSim:
             return X + c
         raise ValueError(self.model)
 
-    def scheme(self, X, c, z):
+    def scheme(self, X, c, z, stim=0.0):
         """One integrator step (simulator/integrators.py:286-426, :496-508).
 
         ``z`` holds the standard normals of this step; noise is
         ``sqrt(dt) * z`` (noise.py:195-198) scaled by ``gfun = sqrt(2 nsig)``
-        (noise.py:223-232)."""
+        (noise.py:223-232); ``stim`` is the (nsvar, N, 1) stimulus array of
+        the step (zero rows outside stvar, simulator.py:297, :306)."""
         dt = self.dt
         f = self.dfun
         if self.integrator == "HeunDeterministic":        # :300-318
             m = f(X, c)
-            inter = X + dt * (m + 0.0)
+            inter = X + dt * (m + stim)
             self._bound(inter)
             dX = (m + f(inter, c)) * dt / 2.0
-            Xn = X + dX + dt * 0.0
+            Xn = X + dX + dt * stim
         elif self.integrator == "HeunStochastic":         # :330-362
             noise = np.sqrt(dt) * z
             m = f(X, c)
             noise = noise * np.sqrt(2.0 * self.nsig)
-            inter = X + dt * m + noise + dt * 0.0
+            inter = X + dt * m + noise + dt * stim
             self._bound(inter)
             dX = (m + f(inter, c)) * dt / 2.0
-            Xn = X + dX + noise + dt * 0.0
+            Xn = X + dX + noise + dt * stim
         elif self.integrator == "EulerDeterministic":     # :377-391
-            Xn = X + dt * (f(X, c) + 0.0)
+            Xn = X + dt * (f(X, c) + stim)
         elif self.integrator == "EulerStochastic":        # :409-426
             noise = np.sqrt(dt) * z
             dX = f(X, c) * dt
-            Xn = X + dX + np.sqrt(2.0 * self.nsig) * noise + dt * 0.0
+            Xn = X + dX + np.sqrt(2.0 * self.nsig) * noise + dt * stim
         elif self.integrator == "Identity":               # :496-508
-            Xn = f(X, c) + 0.0
+            Xn = f(X, c) + stim
         else:
             raise ValueError(self.integrator)
         self._bound(Xn)
@@ -416,16 +452,23 @@ class OracleSim:
         state = self.current_state
         start = self.current_step + 1
         c = self.coupling_at(start)
+        stim = 0.0
+        if self.stimulus is not None:
+            stim = np.zeros((len(self.svars), self.N, 1))
         for k, step in enumerate(range(start, start + n_steps)):
             z = None
             if self.stochastic:
                 z = np.asarray(noise[k], float).reshape((len(self.svars), self.N, 1))
-            state = self.scheme(state, c, z)
+            if self.stimulus is not None:   # simulator.py:301-306: index relative to the call's first step
+                sp, tp = self.stimulus
+                stim[self.stvar, :, :] = (sp * tp[0, k]).reshape((1, -1, 1))
+            state = self.scheme(state, c, z, stim)
             self.ring[step % self.H] = state[..., 0][self.cvar]   # history.py:203-204
             c = self.coupling_at(step + 1)
             obs = observe(self.model, state, self.voi_names)
             for (tl, xl), mon in zip(outs, self.monitors):
-                r = mon.sample(step, obs)
+                # afferent monitors sample the node coupling (simulator.py:311-316)
+                r = mon.sample(step, c if mon.kind in Monitor.AFFERENT else obs)
                 if r is not None:
                     tl.append(r[0])
                     xl.append(r[1])

@@ -10,7 +10,10 @@ MIRROR_MODELS = {"Generic2dOscillator": S.Generic2dOscillator, "MontbrioPazoRoxi
                  "JansenRit": S.JansenRit}
 MIRROR_SCHEMES = {"HeunDeterministic": S.HeunDeterministic, "HeunStochastic": S.HeunStochastic,
                   "EulerDeterministic": S.EulerDeterministic, "EulerStochastic": S.EulerStochastic}
-MIRROR_MONS = {"Raw": S.Raw, "TemporalAverage": S.TemporalAverage, "Bold": S.Bold}
+MIRROR_MONS = {"Raw": S.Raw, "TemporalAverage": S.TemporalAverage, "Bold": S.Bold, "SubSample": S.SubSample,
+               "GlobalAverage": S.GlobalAverage, "SpatialAverage": S.SpatialAverage,
+               "AfferentCoupling": S.AfferentCoupling,
+               "AfferentCouplingTemporalAverage": S.AfferentCouplingTemporalAverage}
 
 
 def initial_history(model, H, n, seed, spread=1.0):
@@ -28,7 +31,9 @@ def initial_history(model, H, n, seed, spread=1.0):
 def make_case(model, n=16, density=0.3, lengths=(10.0, 60.0), speed=3.0, dt=0.1,
               scheme="HeunDeterministic", coupling="Linear", cparams=None, mparams=None,
               nsig=None, seed=0, monitors=(("Raw", None, None),), spread=1.0, weights=None,
-              tract_lengths=None):
+              tract_lengths=None, stimulus=None):
+    """``stimulus``: (weights (N,), temporal equation with ``evaluate(t)``,
+    simulation_length ms) for a region stimulus in both expressions."""
     if weights is None:
         w, L = synthetic(n, density, lengths[0], lengths[1], seed)
     else:
@@ -42,14 +47,22 @@ def make_case(model, n=16, density=0.3, lengths=(10.0, 60.0), speed=3.0, dt=0.1,
               integrator=scheme, initial_conditions=ic, monitors=monitors)
     if nsig is not None:
         kw["nsig"] = nsig
+    stim_obj = None
+    if stimulus is not None:
+        sw, teq, length = stimulus
+        stim_obj = S.StimuliRegion(temporal=teq, weight=np.asarray(sw, float))
+        stim_obj.configure_space()
+        stim_obj.configure_time(np.r_[0.0:float(length):dt].reshape((1, -1)))
+        kw["stimulus"] = (stim_obj.spatial_pattern[:, 0], stim_obj.temporal_pattern[0])
     orc = {mode: OracleSim(w, L, speed, dt, model, mode=mode, **kw) for mode in ("faithful", "fp64")}
     # mirror simulator
     integ = MIRROR_SCHEMES[scheme](dt=dt)
     if scheme.endswith("Stochastic"):
         integ.noise = S.Additive(nsig=np.asarray(nsig if nsig is not None else 1.0, float))
     mons = []
-    for kind, period, voi in monitors:
-        kw2 = {}
+    for spec_m in monitors:
+        kind, period, voi = spec_m[:3]
+        kw2 = dict(spec_m[3]) if len(spec_m) > 3 else {}
         if period is not None:
             kw2["period"] = period
         if voi is not None:
@@ -58,7 +71,9 @@ def make_case(model, n=16, density=0.3, lengths=(10.0, 60.0), speed=3.0, dt=0.1,
     cfun = S.Linear(**cparams) if coupling == "Linear" else S.SigmoidalJansenRit(**cparams)
     sim = S.Simulator(connectivity=S.Connectivity(weights=w, tract_lengths=L, speed=np.array([speed])),
                       model=MIRROR_MODELS[model](**mparams), coupling=cfun, integrator=integ,
-                      monitors=mons, initial_conditions=ic).configure()
+                      monitors=mons, initial_conditions=ic, stimulus=stim_obj).configure()
+    if stimulus is not None:
+        sim.simulation_length = float(stimulus[2])
     return orc, sim, ic
 
 

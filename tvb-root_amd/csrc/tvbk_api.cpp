@@ -141,6 +141,17 @@ struct tvbk_plan {
   DevBuf d_tiles, d_conns, d_edges, d_w, d_rp, d_ring, d_img, d_bacc, d_params, d_flags, d_streams, d_z, d_srec;
   DevBuf d_taps[TVBK_MAX_MON], d_out[TVBK_MAX_MON];
   DevBuf d_bstock[TVBK_MAX_MON];   // Bold monitors with keep_stock: interim-mean rings
+  // monitor kinds as given (d.mon[m].kind holds the kernel's base kind:
+  // RAW, TAVG or BOLD), sample source, areas of GAVG / SAVG
+  int ukind[TVBK_MAX_MON] = {0}, src[TVBK_MAX_MON] = {0}, n_area[TVBK_MAX_MON] = {0};
+  DevBuf d_raw[TVBK_MAX_MON], d_area_of[TVBK_MAX_MON], d_area_w[TVBK_MAX_MON], d_area_jobs[TVBK_MAX_MON];
+  std::vector<AreaJob> jobs_host[TVBK_MAX_MON];
+  // region stimulus of the next run
+  DevBuf d_stim_sp, d_stim_tp;
+  const double* stim_sp = nullptr;
+  const double* stim_tp = nullptr;
+  int64_t stim_steps = 0;
+  uint32_t stim_mask = 0;
   DevBuf d_dense;             // dense zero-delay kernel: W [16*RB][KP] float
   int dense_kb = 0, dense_rb = 0, lds_x = 0;
   int64_t z_steps = 0;
@@ -230,7 +241,13 @@ static void destroy(tvbk_plan* p) {
     dev_free(&p->d_taps[m]);
     dev_free(&p->d_out[m]);
     dev_free(&p->d_bstock[m]);
+    dev_free(&p->d_raw[m]);
+    dev_free(&p->d_area_of[m]);
+    dev_free(&p->d_area_w[m]);
+    dev_free(&p->d_area_jobs[m]);
   }
+  dev_free(&p->d_stim_sp);
+  dev_free(&p->d_stim_tp);
   dev_free(&p->d_dense);
   if (p->ev0) hipEventDestroy(p->ev0);
   if (p->ev1) hipEventDestroy(p->ev1);
@@ -384,14 +401,52 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
     if (inst_conn[b] < 0 || inst_conn[b] >= d.n_conn) { p->err = "inst_conn out of range"; return bail(TVBK_EINVAL); }
     if (inst_conn[b] != inst_conn[0]) shared = false;
   }
+  std::vector<int32_t> area_of[TVBK_MAX_MON];
+  std::vector<double> area_w[TVBK_MAX_MON];
   for (int m = 0; m < d.n_mon; ++m) {
-    const tvbk_monitor& mo = d.mon[m];
-    if (mo.kind < TVBK_MON_RAW || mo.kind > TVBK_MON_BOLD || mo.istep <= 0 || mo.n_voi <= 0 || mo.n_voi > TVBK_MAX_OBS) {
+    tvbk_monitor& mo = p->d.mon[m];
+    const int uk = mo.kind;
+    if (uk < TVBK_MON_RAW || uk > TVBK_MON_AFF_TAVG || mo.istep <= 0 || mo.n_voi <= 0 || mo.n_voi > TVBK_MAX_OBS) {
       p->err = "bad monitor " + std::to_string(m);
       return bail(TVBK_EINVAL);
     }
-    for (int v = 0; v < mo.n_voi; ++v)
-      if (mo.voi[v] < 0 || mo.voi[v] >= d.n_obs) { p->err = "monitor voi out of range"; return bail(TVBK_EINVAL); }
+    p->ukind[m] = uk;
+    p->src[m] = (uk == TVBK_MON_AFF || uk == TVBK_MON_AFF_TAVG) ? 1 : 0;
+    mo.kind = (uk == TVBK_MON_TAVG || uk == TVBK_MON_AFF_TAVG) ? TVBK_MON_TAVG
+              : uk == TVBK_MON_BOLD                          ? TVBK_MON_BOLD
+                                                              : TVBK_MON_RAW;
+    if (uk == TVBK_MON_RAW && mo.istep != 1) { p->err = "Raw samples every step (istep 1)"; return bail(TVBK_EINVAL); }
+    // afferent monitors index the model's coupling variables; the kernel's
+    // coupling terms (one for SigmoidalJansenRit, whose two reference rows
+    // are equal, coupling.py:358-376) hold them
+    const int nvsrc = p->src[m] ? p->mi.ncv : d.n_obs;
+    for (int v = 0; v < mo.n_voi; ++v) {
+      if (mo.voi[v] < 0 || mo.voi[v] >= nvsrc) { p->err = "monitor voi out of range"; return bail(TVBK_EINVAL); }
+      if (p->src[m]) mo.voi[v] = std::min(mo.voi[v], p->nct - 1);
+    }
+    if (uk == TVBK_MON_GAVG) {
+      p->n_area[m] = 1;
+    } else if (uk == TVBK_MON_SAVG) {
+      const int A = mo.n_area;
+      if (d.n_conn != 1 || !mo.spatial_mask || A <= 0) {
+        p->err = "SpatialAverage needs one connectome, a spatial mask and n_area > 0";
+        return bail(TVBK_EINVAL);
+      }
+      const int N0 = d.conns[0].n_node;
+      std::vector<int> cnt(A, 0);
+      area_of[m].assign(mo.spatial_mask, mo.spatial_mask + N0);
+      for (int n = 0; n < N0; ++n) {
+        if (area_of[m][n] < 0 || area_of[m][n] >= A) { p->err = "spatial mask value out of range"; return bail(TVBK_EINVAL); }
+        ++cnt[area_of[m][n]];
+      }
+      for (int q = 0; q < A; ++q)   // monitors.py:315-320: contiguous area indices from zero
+        if (cnt[q] == 0) { p->err = "every area of the spatial mask needs a node"; return bail(TVBK_EINVAL); }
+      // spatial_mean = spatial_sum / nodes_per_area (monitors.py:325-331)
+      area_w[m].resize(N0);
+      for (int n = 0; n < N0; ++n) area_w[m][n] = 1.0 / (double)cnt[area_of[m][n]];
+      p->n_area[m] = A;
+    }
+    if (mo.kind == TVBK_MON_BOLD && uk != TVBK_MON_BOLD) { p->err = "bad monitor kind"; return bail(TVBK_EINVAL); }
     if (mo.kind == TVBK_MON_BOLD) {
       if (mo.interim_istep <= 0 || mo.n_taps <= 0 || !mo.taps) { p->err = "bad Bold monitor"; return bail(TVBK_EINVAL); }
       // live accumulators: emissions within S interim samples of a sample
@@ -408,17 +463,20 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   max_lds -= 2048;   // the kernels' static LDS (tile and monitor configuration)
   int kernel = d.kernel;
   // dense zero-delay kernel: every delay zero (horizon 1), one shared
-  // connectome of at most 128 nodes, Linear coupling, float32, G2D / MPR
+  // connectome of at most 128 nodes, Linear coupling, float32, G2D / MPR,
+  // no afferent monitor
+  bool any_aff = false;
+  for (int m = 0; m < d.n_mon; ++m) any_aff |= p->src[m] != 0;
   const bool dense_ok = shared && d.precision != TVBK_F64 && d.cfun == TVBK_CFUN_LINEAR &&
                         (d.model == TVBK_MODEL_G2D || d.model == TVBK_MODEL_MPR) && p->conns.size() == 1 &&
-                        p->conns[0].horizon == 1 && p->conns[0].n_node <= 128;
+                        p->conns[0].horizon == 1 && p->conns[0].n_node <= 128 && !any_aff;
   if (kernel == TVBK_KERNEL_AUTO) {
     if (dense_ok && d.n_inst >= 32) kernel = TVBK_KERNEL_DENSE_MFMA;
     else kernel = (shared && d.n_inst >= 32) ? TVBK_KERNEL_BATCH_LANE : TVBK_KERNEL_EDGE_LANE;
   }
   if (kernel == TVBK_KERNEL_DENSE_MFMA && !dense_ok) {
     p->err = "dense MFMA kernel needs a float32 history, Linear coupling, G2D or MPR, one connectome of <= 128 nodes "
-             "with all delays zero";
+             "with all delays zero, and no afferent monitor";
     return bail(TVBK_EUNSUPPORTED);
   }
   if (kernel == TVBK_KERNEL_BATCH_LANE && !shared) { p->err = "batch-lane kernel needs one shared connectome"; return bail(TVBK_EINVAL); }
@@ -586,6 +644,9 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
       else
         for (int b = 0; b < B; ++b) elems += cap * d.mon[m].n_voi * (int64_t)p->conns[inst_conn[b]].n_node;
       tot += (size_t)elems * (d.mon[m].kind == TVBK_MON_BOLD ? 8 : p->esize);
+      if (p->n_area[m] > 0)   // node-level samples, then the per-area output
+        tot += (size_t)(cap * d.mon[m].n_voi * p->n_area[m] * B) * p->esize +
+               (size_t)p->conns[0].n_node * 12 + (size_t)B * sizeof(AreaJob);
     }
     if (d.noise == TVBK_NOISE_INJECTED) tot += (size_t)d.max_steps * p->mi.nsv * p->max_n * B * 8;
     if (footprint) *footprint = (int64_t)tot;
@@ -695,7 +756,19 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
     } else {
       for (int b = 0; b < B; ++b) elems += cap * d.mon[m].n_voi * (int64_t)p->conns[inst_conn[b]].n_node;
     }
-    if ((rc = dev_alloc(p, &p->d_out[m], (size_t)elems * es))) return bail(rc);
+    if (p->n_area[m] > 0) {
+      // GlobalAverage / SpatialAverage: the step kernel subsamples the nodes
+      // into d_raw, k_areas reduces them into d_out
+      if ((rc = dev_alloc(p, &p->d_raw[m], (size_t)elems * es))) return bail(rc);
+      if ((rc = dev_alloc(p, &p->d_out[m], (size_t)(cap * d.mon[m].n_voi * p->n_area[m] * B) * es))) return bail(rc);
+      if ((rc = dev_alloc(p, &p->d_area_jobs[m], (size_t)B * sizeof(AreaJob)))) return bail(rc);
+      if (p->ukind[m] == TVBK_MON_SAVG) {
+        if ((rc = up(&p->d_area_of[m], area_of[m].data(), area_of[m].size() * 4))) return bail(rc);
+        if ((rc = up(&p->d_area_w[m], area_w[m].data(), area_w[m].size() * 8))) return bail(rc);
+      }
+    } else if ((rc = dev_alloc(p, &p->d_out[m], (size_t)elems * es))) {
+      return bail(rc);
+    }
     p->mon_count[m].assign(B, 0);
     p->mon_first[m].assign(B, 0);
     p->mon_offset[m].assign(B, 0);
@@ -893,6 +966,39 @@ int tvbk_set_noise(tvbk_plan* p, const double* z, int64_t n_steps) {
   return TVBK_OK;
 }
 
+int tvbk_set_stimulus(tvbk_plan* p, const double* spatial, const double* temporal, int64_t n_steps,
+                      uint32_t svar_mask) {
+  if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
+  HIPCHK(p, hipStreamSynchronize(p->stream));   // the previous run may still read the buffers
+  if (!temporal) {
+    p->stim_tp = nullptr;
+    p->stim_steps = 0;
+    return TVBK_OK;
+  }
+  if (!spatial || n_steps <= 0) return fail(p, TVBK_EINVAL, "empty stimulus");
+  if (p->kernel == TVBK_KERNEL_DENSE_MFMA) return fail(p, TVBK_EUNSUPPORTED, "the dense kernel takes no stimulus");
+  if (p->conns.size() != 1) return fail(p, TVBK_EUNSUPPORTED, "a region stimulus needs one connectome");
+  if (svar_mask == 0 || (svar_mask >> p->mi.nsv) != 0) return fail(p, TVBK_EINVAL, "bad stimulus state-variable mask");
+  const size_t nsp = (size_t)p->conns[0].n_node * 8, ntp = (size_t)n_steps * 8;
+  int rc;
+  if (p->d_stim_sp.bytes < nsp) {
+    dev_free(&p->d_stim_sp);
+    if ((rc = dev_alloc(p, &p->d_stim_sp, nsp))) return rc;
+  }
+  if (p->d_stim_tp.bytes < ntp) {
+    dev_free(&p->d_stim_tp);
+    if ((rc = dev_alloc(p, &p->d_stim_tp, ntp))) return rc;
+  }
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  HIPCHK(p, hipMemcpy(p->d_stim_sp.p, spatial, nsp, hipMemcpyDefault));
+  HIPCHK(p, hipMemcpy(p->d_stim_tp.p, temporal, ntp, hipMemcpyDefault));
+  p->stim_sp = (const double*)p->d_stim_sp.p;
+  p->stim_tp = (const double*)p->d_stim_tp.p;
+  p->stim_steps = n_steps;
+  p->stim_mask = svar_mask;
+  return TVBK_OK;
+}
+
 int tvbk_set_steps(tvbk_plan* p, const int64_t* remaining) {
   if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
   for (int b = 0; b < p->d.n_inst; ++b) p->remaining[b] = remaining ? remaining[b] : -1;
@@ -955,10 +1061,26 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
   for (int m = 0; m < d.n_mon; ++m) {
     const int es = d.mon[m].kind == TVBK_MON_BOLD ? 8 : p->esize;
     int64_t cnt = 0;
-    if (T > 1) cnt = p->mon_count[m][0] * d.mon[m].n_voi * (int64_t)p->conns[0].n_node * B;
-    else cnt = out_off[m];
+    if (p->n_area[m] > 0) {
+      // reduced output: [sample][voi][area][inst] (T > 1) or per instance
+      // [sample][voi][area] one after the other
+      const int A = p->n_area[m], nv = d.mon[m].n_voi;
+      if (T > 1) {
+        cnt = p->mon_count[m][0] * nv * (int64_t)A * B;
+      } else {
+        for (int b = 0; b < B; ++b) {
+          p->mon_offset[m][b] = cnt;
+          cnt += p->mon_count[m][b] * nv * (int64_t)A;
+        }
+      }
+    } else if (T > 1) {
+      cnt = p->mon_count[m][0] * d.mon[m].n_voi * (int64_t)p->conns[0].n_node * B;
+    } else {
+      cnt = out_off[m];
+    }
     p->mon_bytes[m] = cnt * es;
   }
+  if (p->stim_tp && p->stim_steps < n) return fail(p, TVBK_ESTATE, "the stimulus set for this run is shorter than it");
   HIPCHK(p, hipMemcpyAsync(p->d_tiles.p, p->tiles.data(), p->tiles.size() * sizeof(TileDesc), hipMemcpyHostToDevice,
                            p->stream));
   HIPCHK(p, hipMemsetAsync(p->d_flags.p, 0, (size_t)B * 4, p->stream));
@@ -972,7 +1094,11 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
   a.ring = p->d_ring.p;
   a.img = (unsigned char*)p->d_img.p;
   a.bacc = (double*)p->d_bacc.p;
-  for (int m = 0; m < d.n_mon; ++m) a.out[m] = p->d_out[m].p;
+  for (int m = 0; m < d.n_mon; ++m) a.out[m] = p->n_area[m] > 0 ? p->d_raw[m].p : p->d_out[m].p;
+  a.stim_sp = p->stim_tp ? p->stim_sp : nullptr;
+  a.stim_tp = p->stim_tp;
+  a.stim_mask = p->stim_mask;
+  for (int m = 0; m < d.n_mon; ++m) a.has_aff |= p->src[m];
   a.z = p->z_ptr;
   a.params = p->d_params.p;
   a.flags = (int32_t*)p->d_flags.p;
@@ -1030,6 +1156,7 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
     ma.k1v0 = mo.k1_v0;
     ma.taps = (const double*)p->d_taps[m].p;
     ma.stock = (double*)p->d_bstock[m].p;
+    ma.src = p->src[m];
     ma.lds_off = p->mon_lds_off[m];
   }
   a.dense_w = (const float*)p->d_dense.p;
@@ -1043,6 +1170,47 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
   else
     HIPCHK(p, launch_step(d.precision, d.model, d.cfun, a, (int)p->tiles.size(), p->nthreads, (size_t)p->lds_total,
                           p->stream));
+  // GlobalAverage / SpatialAverage of the call's node samples
+  for (int m = 0; m < d.n_mon; ++m) {
+    if (p->n_area[m] <= 0) continue;
+    const int nv = d.mon[m].n_voi;
+    std::vector<AreaJob>& jobs = p->jobs_host[m];
+    jobs.assign(B, AreaJob{});
+    int64_t maxc = 0;
+    for (size_t k = 0; k < p->tiles.size(); ++k) {
+      const TileDesc& td = p->tiles[k];
+      const int Nt = p->conns[td.conn].n_node;
+      for (int l = 0; l < td.n_inst; ++l) {
+        const int b = td.inst0 + l;
+        AreaJob& jb = jobs[b];
+        jb.n_node = Nt;
+        jb.count = (int32_t)p->mon_count[m][b];
+        maxc = std::max<int64_t>(maxc, jb.count);
+        jb.raw_off = td.out_off[m] + (T > 1 ? l : 0);
+        jb.rsj = td.out_sj[m];
+        jb.rsv = td.out_sv[m];
+        jb.rsn = td.out_si[m];
+        const int A = p->n_area[m];
+        if (T > 1) {
+          jb.out_off = b;
+          jb.osa = B;
+          jb.osv = (int64_t)A * B;
+          jb.osj = (int64_t)nv * A * B;
+        } else {
+          jb.out_off = p->mon_offset[m][b];
+          jb.osa = 1;
+          jb.osv = A;
+          jb.osj = (int64_t)nv * A;
+        }
+      }
+    }
+    if (maxc == 0) continue;
+    HIPCHK(p, hipMemcpyAsync(p->d_area_jobs[m].p, jobs.data(), jobs.size() * sizeof(AreaJob), hipMemcpyHostToDevice,
+                             p->stream));
+    HIPCHK(p, launch_areas(p->esize, p->ukind[m] == TVBK_MON_SAVG ? 1 : 0, p->d_raw[m].p, p->d_out[m].p,
+                           (const AreaJob*)p->d_area_jobs[m].p, B, (int)maxc, nv, p->n_area[m],
+                           (const int32_t*)p->d_area_of[m].p, (const double*)p->d_area_w[m].p, p->stream));
+  }
   HIPCHK(p, hipEventRecord(p->ev1, p->stream));
   p->have_timing = true;
   for (size_t k = 0; k < p->tiles.size(); ++k) {
@@ -1055,6 +1223,8 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
   }
   p->last_n = n;
   if (d.noise == TVBK_NOISE_INJECTED) p->z_steps = 0;
+  p->stim_tp = nullptr;   // a stimulus is indexed by the call's steps: set it again for the next call
+  p->stim_steps = 0;
   return TVBK_OK;
 }
 

@@ -378,10 +378,14 @@ __device__ inline void philox_normals(int nsv, uint32_t node, uint64_t stream, i
 
 // ---------------------------------------------------------------------------
 // Integrator schemes (integrators.py:286-426, :496-508).  xi[sv] is the noise
-// increment sqrt(dt)*z*gfun of this step (zero for deterministic schemes).
+// increment sqrt(dt)*z*gfun of this step (zero for deterministic schemes);
+// with STIM, st[sv] is the region stimulus of this step, entering each scheme
+// where the reference adds `stimulus` (:310, :315, :354, :359, :388, :423,
+// :506), in the same operation order.
 // ---------------------------------------------------------------------------
-template <class R, class M>
-__device__ inline void scheme_step(int sch, R* x, const R* c, const R* p, R dt, const R* xi) {
+template <class R, class M, bool STIM = false>
+__device__ inline void scheme_step(int sch, R* x, const R* c, const R* p, R dt, const R* xi,
+                                   const R* st = nullptr) {
   constexpr int S = M::NSV;
   R m[S], y[S], m2[S];
   M::dfun(x, c, p, m);
@@ -390,27 +394,42 @@ __device__ inline void scheme_step(int sch, R* x, const R* c, const R* p, R dt, 
     case SCH_HEUN_STO: {
       const bool sto = (sch == SCH_HEUN_STO);
 #pragma unroll
-      for (int k = 0; k < S; ++k) y[k] = sto ? (x[k] + dt * m[k]) + xi[k] : x[k] + dt * m[k];
+      for (int k = 0; k < S; ++k) {
+        if constexpr (STIM)
+          y[k] = sto ? ((x[k] + dt * m[k]) + xi[k]) + dt * st[k] : x[k] + dt * (m[k] + st[k]);
+        else
+          y[k] = sto ? (x[k] + dt * m[k]) + xi[k] : x[k] + dt * m[k];
+      }
       M::template bound<R>(y);
       M::dfun(y, c, p, m2);
 #pragma unroll
       for (int k = 0; k < S; ++k) {
         const R dX = ((m[k] + m2[k]) * dt) * R(0.5);
         x[k] = sto ? (x[k] + dX) + xi[k] : x[k] + dX;
+        if constexpr (STIM) x[k] = x[k] + dt * st[k];
       }
       break;
     }
     case SCH_EULER_DET:
 #pragma unroll
-      for (int k = 0; k < S; ++k) x[k] = x[k] + dt * m[k];
+      for (int k = 0; k < S; ++k) {
+        if constexpr (STIM) x[k] = x[k] + dt * (m[k] + st[k]);
+        else x[k] = x[k] + dt * m[k];
+      }
       break;
     case SCH_EULER_STO:
 #pragma unroll
-      for (int k = 0; k < S; ++k) x[k] = (x[k] + m[k] * dt) + xi[k];
+      for (int k = 0; k < S; ++k) {
+        x[k] = (x[k] + m[k] * dt) + xi[k];
+        if constexpr (STIM) x[k] = x[k] + dt * st[k];
+      }
       break;
     default:  // identity
 #pragma unroll
-      for (int k = 0; k < S; ++k) x[k] = m[k];
+      for (int k = 0; k < S; ++k) {
+        if constexpr (STIM) x[k] = m[k] + st[k];
+        else x[k] = m[k];
+      }
       break;
   }
   M::template bound<R>(x);

@@ -1,6 +1,8 @@
 // Small kernels (history scatter/gather, slot arithmetic check) and the
 // dispatch from the plan's runtime (precision, model, coupling) to the
 // instantiated step, init and dense kernels of tvbk_inst.hip.
+#include <algorithm>
+
 #include "tvbk_kernels.hpp"
 #include "tvbk_device.hpp"
 
@@ -102,6 +104,87 @@ hipError_t launch_scatter(int precision, bool gather, void* ring, unsigned char*
   else
     hipLaunchKernelGGL((k_scatter<float, float>), dim3(64), dim3(256), 0, st, gather, (float*)ring,
                        (float*)img_state, (float*)hist, (float*)state, H, N, nch, nsv, T, t);
+  return hipGetLastError();
+}
+
+// numpy's pairwise summation (the add.reduce inner loop over a contiguous
+// axis): under 8 values a plain sum from 0, up to 128 eight interleaved
+// partial sums combined as ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)) and the rest
+// added in order, beyond that the two halves (the first a multiple of 8) summed
+// separately.  Templated on the remaining split depth (2^12 * 128 values).
+template <class R, int D>
+struct PairwiseSum {
+  __device__ static R run(const R* x, int64_t st, int64_t n) {
+    if (n <= 128) return PairwiseSum<R, 0>::run(x, st, n);
+    int64_t n2 = n / 2;
+    n2 -= n2 % 8;
+    return PairwiseSum<R, D - 1>::run(x, st, n2) + PairwiseSum<R, D - 1>::run(x + n2 * st, st, n - n2);
+  }
+};
+template <class R>
+struct PairwiseSum<R, 0> {
+  __device__ static R run(const R* x, int64_t st, int64_t n) {
+    if (n < 8) {
+      R res = R(0);
+      for (int64_t i = 0; i < n; ++i) res = res + x[i * st];
+      return res;
+    }
+    R r[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = x[j * st];
+    int64_t i = 8;
+    for (; i < n - (n % 8); i += 8)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) r[j] = r[j] + x[(i + j) * st];
+    R res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    for (; i < n; ++i) res = res + x[i * st];
+    return res;
+  }
+};
+
+// GlobalAverage (monitors.py:352-357: numpy.mean over the node axis) and
+// SpatialAverage (:328-333: spatial_mean . state, each area's members in node
+// order) of subsampled monitor data, one thread per (sample, voi, area) of an
+// instance (blockIdx.y strides over instances).
+template <class R>
+__global__ void k_areas(int mode, const R* __restrict__ raw, R* __restrict__ out, const AreaJob* __restrict__ jobs,
+                        int n_jobs, int max_count, int nvoi, int n_area, const int32_t* __restrict__ area_of,
+                        const double* __restrict__ weight) {
+  const int64_t per = (int64_t)max_count * nvoi * n_area;
+  for (int b = blockIdx.y; b < n_jobs; b += gridDim.y) {
+    const AreaJob jb = jobs[b];
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < per; q += (int64_t)gridDim.x * blockDim.x) {
+      const int a = (int)(q % n_area);
+      const int v = (int)((q / n_area) % nvoi);
+      const int j = (int)(q / ((int64_t)n_area * nvoi));
+      if (j >= jb.count) continue;
+      const R* xs = raw + jb.raw_off + (int64_t)j * jb.rsj + (int64_t)v * jb.rsv;
+      R res;
+      if (mode == 0) {
+        res = PairwiseSum<R, 12>::run(xs, jb.rsn, jb.n_node) / (R)jb.n_node;
+      } else {
+        res = R(0);
+        for (int n = 0; n < jb.n_node; ++n)
+          if (area_of[n] == a) res = res + (R)weight[n] * xs[(int64_t)n * jb.rsn];
+      }
+      out[jb.out_off + (int64_t)j * jb.osj + (int64_t)v * jb.osv + (int64_t)a * jb.osa] = res;
+    }
+  }
+}
+
+hipError_t launch_areas(int esize, int mode, const void* raw, void* out, const AreaJob* jobs, int n_jobs,
+                        int max_count, int nvoi, int n_area, const int32_t* area_of, const double* weight,
+                        hipStream_t st) {
+  if (n_jobs <= 0 || max_count <= 0) return hipSuccess;
+  const int64_t per = (int64_t)max_count * nvoi * n_area;
+  const int bx = (int)std::min<int64_t>((per + 255) / 256, 4096);
+  const int by = std::min(n_jobs, 65535);
+  if (esize == 8)
+    hipLaunchKernelGGL(k_areas<double>, dim3(bx, by), dim3(256), 0, st, mode, (const double*)raw, (double*)out, jobs,
+                       n_jobs, max_count, nvoi, n_area, area_of, weight);
+  else
+    hipLaunchKernelGGL(k_areas<float>, dim3(bx, by), dim3(256), 0, st, mode, (const float*)raw, (float*)out, jobs,
+                       n_jobs, max_count, nvoi, n_area, area_of, weight);
   return hipGetLastError();
 }
 

@@ -32,7 +32,8 @@ struct TileDesc {
 
 struct MonArgs {
   int32_t kind, istep, nvoi, ii, S, Q, sub1, lds_off;
-  int32_t rat, pad;   // Bold: istep / ii when ii divides istep, else 0
+  int32_t rat;        // Bold: istep / ii when ii divides istep, else 0
+  int32_t src;        // sample source: 0 observed state, 1 node coupling (afferent)
   int32_t voi[TVBK_KMAX_OBS];
   double k1v0;
   const double* taps;
@@ -80,7 +81,22 @@ struct StepArgs {
   int64_t img_bytes;
   int32_t obs_sv[TVBK_KMAX_OBS];
   double obs_coef[TVBK_KMAX_OBS][8];
+  // region stimulus of this call: spatial[node] * temporal[k] for the state
+  // variables in stim_mask (null temporal: none); afferent monitors present
+  const double* stim_sp;
+  const double* stim_tp;
+  uint32_t stim_mask;
+  int32_t has_aff;
   MonArgs mon[TVBK_KMAX_MON];
+};
+
+// Node -> area reduction of one monitor's subsampled output (GlobalAverage,
+// SpatialAverage): per instance b, samples j < count, voi v, area a.
+struct AreaJob {
+  int64_t raw_off, out_off;   // element offsets of instance b's data
+  int64_t rsj, rsv, rsn;      // raw strides: sample, voi, node
+  int64_t osj, osv, osa;      // output strides: sample, voi, area
+  int32_t n_node, count;
 };
 
 struct InitArgs {
@@ -99,6 +115,11 @@ hipError_t launch_step(int precision, int model, int cf, const StepArgs& a, int 
                        hipStream_t st);
 hipError_t launch_init(int precision, int model, int cf, const InitArgs& a, int ntiles, hipStream_t st);
 hipError_t launch_dense(int precision, int model, const StepArgs& a, int ntiles, int nthreads, size_t lds,
+                        hipStream_t st);
+// mode 0: mean over all nodes (numpy pairwise order); 1: sum over each
+// area's nodes of weight[node] * x, in node order (area_of[node])
+hipError_t launch_areas(int esize, int mode, const void* raw, void* out, const AreaJob* jobs, int n_jobs,
+                        int max_count, int nvoi, int n_area, const int32_t* area_of, const double* weight,
                         hipStream_t st);
 hipError_t launch_debug_slots(const uint32_t* edges, int nnz, int H, int64_t step, int32_t* out, hipStream_t st);
 // Balloon-Windkessel BOLD, one thread per series

@@ -106,7 +106,9 @@ template <> __device__ inline double buf_load2<double>(__amdgpu_buffer_rsrc_t r,
 __device__ inline void step_info(const MonArgs& mo, int k, StepInfo& si) {
   const int ek = mo.r0 + k;
   si.emit = (ek % mo.istep) == 0;
-  si.msample = ek / mo.istep - (mo.r0 > 0 ? 1 : 0);
+  // sample index among the call's emissions; afferent monitors keep the
+  // phase of step s0 - 1 (their samples of step s are taken at step s + 1)
+  si.msample = ek / mo.istep - (mo.src ? 1 : (mo.r0 > 0 ? 1 : 0));
   si.kk = mo.qe0 + ek / mo.istep;
   const int pk = mo.rb0 + k;
   si.push = (pk % mo.ii) == 0;
@@ -141,69 +143,47 @@ __device__ inline void stage_params(const StepArgs& a, unsigned char* smem, int 
   }
 }
 
-// Integrator step, history write and monitors of one (node i, lane t) pair
-// with its coupling terms c (phase B of the step kernels).  `sink(ch, v)`
-// stores history channel ch of the new state.
-template <class R, class M, int CF, class NR, bool UM = false, class SINK>
-__device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, unsigned char* smem, R* s_x, int i,
-                                            int t, int T, int N, bool live, int inst, uint64_t stream, int k,
-                                            int64_t s, int buf, const R* par, int pst, const R* c,
-                                            int& bad, SINK sink) {
+// Monitors of one (node i, lane t) pair after step k (monitors.py:160-190,
+// :205-214, :363-431, :834-962): samples of the observed state x, or of the
+// node coupling c (afferent monitors, which sample the coupling of step
+// s + 1 at step s, simulator.py:311-316).  EXT: afferent monitors may be
+// present; aff_only: the pass after the last step, afferent monitors only.
+template <class R, class M, int CF, bool EXT>
+__device__ __forceinline__ void monitors_node(const StepArgs& a, const KCfg& kc, unsigned char* smem, int i, int t,
+                                              int T, int N, bool live, int k, int buf, const R* x, const R* c,
+                                              bool aff_only) {
   using C = Cfun<M, CF>;
-  constexpr int NSV = M::NSV, NDP = M::NDP, NCP = C::NCP, NCH = C::NCH;
+  constexpr int NSV = M::NSV, NCT = C::NCT;
   // this tile's descriptor: read-only during the kernel, scalar loads
   const auto& td = *(const __attribute__((address_space(4))) TileDesc*)(a.tiles + blockIdx.x);
-  const R dt = (R)a.dt;
-  const int64_t spatial_row = (int64_t)N * a.prow;
-  R x[NSV], xi[NSV];
+  auto sample_val = [&](const MonArgs& mo, int v) -> R {
+    const int o = mo.voi[v];
+    R val;
+    if (EXT && mo.src) {       // coupling term o of this node
+      val = c[0];
 #pragma unroll
-  for (int v = 0; v < NSV; ++v) x[v] = s_x[(i * NSV + v) * T + t];
-  // this lane's parameters (registers, stride 1, or LDS, stride T): model,
-  // coupling, noise gfun
-  R pp[NDP > 0 ? NDP : 1], cp[NCP > 0 ? NCP : 1], gf[NSV];
-  // model parameters: plan constants from the kernel arguments (UM), else
-  // this lane's
+      for (int u = 1; u < NCT; ++u) val = (u == o) ? c[u] : val;
+      return val;
+    }
+    const int osv = a.obs_sv[o];
+    if (osv >= 0) {
+      val = x[0];
 #pragma unroll
-  for (int v = 0; v < NDP; ++v) pp[v] = UM ? (R)a.par_uval[v] : lane_param<R>(a, par, pst, v);
+      for (int u = 1; u < NSV; ++u) val = (u == osv) ? x[u] : val;
+    } else {
+      val = R(0);
 #pragma unroll
-  for (int v = 0; v < NCP; ++v) cp[v] = lane_param<R>(a, par, pst, NDP + v);
-#pragma unroll
-  for (int v = 0; v < NSV; ++v) gf[v] = (a.noise != NOISE_NONE) ? lane_param<R>(a, par, pst, NDP + NCP + v) : R(0);
-  if (a.spatial) {
-    const R* pr = reinterpret_cast<const R*>(a.params) + (int64_t)inst * spatial_row + (int64_t)i * a.prow;
-#pragma unroll
-    for (int v = 0; v < NDP; ++v) pp[v] = pr[v];
-#pragma unroll
-    for (int v = 0; v < NSV; ++v) gf[v] = (a.noise != NOISE_NONE) ? pr[NDP + NCP + v] : R(0);
-  }
-  if (a.noise == NOISE_INJECTED) {
-    // padded lanes (t >= n_inst) read lane 0's noise: never past the array
-    const double* z = a.z + td.z_off + k * td.z_sk + (int64_t)i * td.z_si + (live ? t : 0);
-#pragma unroll
-    for (int v = 0; v < NSV; ++v) xi[v] = (R)((a.sqrt_dt * z[v * td.z_ssv]) * (double)gf[v]);
-  } else if (a.noise == NOISE_PHILOX) {
-    // normals in the history precision NR (float for the float32-history
-    // modes), the increment in the state precision
-    NR zz[NSV];
-    philox_normals<NR>(NSV, (uint32_t)i, stream, s, a.key0, a.key1, zz);
-#pragma unroll
-    for (int v = 0; v < NSV; ++v) xi[v] = ((R)a.sqrt_dt * (R)zz[v]) * gf[v];
-  } else {
-#pragma unroll
-    for (int v = 0; v < NSV; ++v) xi[v] = R(0);
-  }
-  scheme_step<R, M>(a.scheme, x, c, pp, dt, xi);
-#pragma unroll
-  for (int v = 0; v < NSV; ++v) {
-    bad |= !isfinite(x[v]);
-    s_x[(i * NSV + v) * T + t] = x[v];
-  }
-#pragma unroll
-  for (int ch = 0; ch < NCH; ++ch) sink(ch, C::template channel<R>(x, ch, cp));
-
-  // ---------------- monitors (monitors.py:160-190, :363-391, :834-962)
+      for (int u = 0; u < NSV; ++u) val = val + (R)a.obs_coef[o][u] * x[u];
+    }
+    return val;
+  };
   for (int m = 0; m < a.n_mon; ++m) {
     const MonArgs& mo = a.mon[m];   // plan constants: kernel-argument (scalar) loads
+    if constexpr (EXT) {
+      // afferent samples of step s - 1 are taken at step s (k >= 1) and by
+      // the tail pass after the loop; the others at their own step
+      if (mo.src ? (k == 0) : aff_only) continue;
+    }
     const StepInfo& si = kc.si[buf][m];
     const int nv = mo.nvoi;
     const bool emit = si.emit != 0;
@@ -214,20 +194,9 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
       };
       R* ts = reinterpret_cast<R*>(smem + mo.lds_off);
       for (int v = 0; v < nv; ++v) {
-        const int o = mo.voi[v];
-        R val;
-        const int osv = a.obs_sv[o];
-        if (osv >= 0) {
-          val = x[0];
-#pragma unroll
-          for (int u = 1; u < NSV; ++u) val = (u == osv) ? x[u] : val;
-        } else {
-          val = R(0);
-#pragma unroll
-          for (int u = 0; u < NSV; ++u) val = val + (R)a.obs_coef[o][u] * x[u];
-        }
+        const R val = sample_val(mo, v);
         if (mo.kind == MON_RAW) {
-          if (live) outp()[v * td.out_sv[m]] = val;
+          if (live && emit) outp()[v * td.out_sv[m]] = val;
         } else {
           // running sum over the averaging window: Kahan-compensated in
           // float32 (sum, compensation), plain sequential (numpy's mean
@@ -257,18 +226,7 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
       const int64_t mp = si.mp;                      // interim sample index s / ii
       double* acc = a.bacc + td.bacc_off[m];
       for (int v = 0; v < nv; ++v) {
-        const int o = mo.voi[v];
-        R val;
-        const int osv = a.obs_sv[o];
-        if (osv >= 0) {
-          val = x[0];
-#pragma unroll
-          for (int u = 1; u < NSV; ++u) val = (u == osv) ? x[u] : val;
-        } else {
-          val = R(0);
-#pragma unroll
-          for (int u = 0; u < NSV; ++u) val = val + (R)a.obs_coef[o][u] * x[u];
-        }
+        const R val = sample_val(mo, v);
         double* bsum = bi + (i * nv + v) * T + t;
         *bsum = *bsum + (double)val;
         if (push) {
@@ -318,6 +276,82 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
   }
 }
 
+// Integrator step, history write and monitors of one (node i, lane t) pair
+// with its coupling terms c (phase B of the step kernels).  `sink(ch, v)`
+// stores history channel ch of the new state.  EXT: region stimulus and
+// afferent monitors may be present (the sequential loop of k_step).
+template <class R, class M, int CF, class NR, bool UM = false, bool EXT = false, class SINK>
+__device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, unsigned char* smem, R* s_x, int i,
+                                            int t, int T, int N, bool live, int inst, uint64_t stream, int k,
+                                            int64_t s, int buf, const R* par, int pst, const R* c,
+                                            int& bad, SINK sink) {
+  using C = Cfun<M, CF>;
+  constexpr int NSV = M::NSV, NDP = M::NDP, NCP = C::NCP, NCH = C::NCH;
+  // this tile's descriptor: read-only during the kernel, scalar loads
+  const auto& td = *(const __attribute__((address_space(4))) TileDesc*)(a.tiles + blockIdx.x);
+  const R dt = (R)a.dt;
+  const int64_t spatial_row = (int64_t)N * a.prow;
+  R x[NSV], xi[NSV];
+#pragma unroll
+  for (int v = 0; v < NSV; ++v) x[v] = s_x[(i * NSV + v) * T + t];
+  // this lane's parameters (registers, stride 1, or LDS, stride T): model,
+  // coupling, noise gfun
+  R pp[NDP > 0 ? NDP : 1], cp[NCP > 0 ? NCP : 1], gf[NSV];
+  // model parameters: plan constants from the kernel arguments (UM), else
+  // this lane's
+#pragma unroll
+  for (int v = 0; v < NDP; ++v) pp[v] = UM ? (R)a.par_uval[v] : lane_param<R>(a, par, pst, v);
+#pragma unroll
+  for (int v = 0; v < NCP; ++v) cp[v] = lane_param<R>(a, par, pst, NDP + v);
+#pragma unroll
+  for (int v = 0; v < NSV; ++v) gf[v] = (a.noise != NOISE_NONE) ? lane_param<R>(a, par, pst, NDP + NCP + v) : R(0);
+  if (a.spatial) {
+    const R* pr = reinterpret_cast<const R*>(a.params) + (int64_t)inst * spatial_row + (int64_t)i * a.prow;
+#pragma unroll
+    for (int v = 0; v < NDP; ++v) pp[v] = pr[v];
+#pragma unroll
+    for (int v = 0; v < NSV; ++v) gf[v] = (a.noise != NOISE_NONE) ? pr[NDP + NCP + v] : R(0);
+  }
+  if (a.noise == NOISE_INJECTED) {
+    // padded lanes (t >= n_inst) read lane 0's noise: never past the array
+    const double* z = a.z + td.z_off + k * td.z_sk + (int64_t)i * td.z_si + (live ? t : 0);
+#pragma unroll
+    for (int v = 0; v < NSV; ++v) xi[v] = (R)((a.sqrt_dt * z[v * td.z_ssv]) * (double)gf[v]);
+  } else if (a.noise == NOISE_PHILOX) {
+    // normals in the history precision NR (float for the float32-history
+    // modes), the increment in the state precision
+    NR zz[NSV];
+    philox_normals<NR>(NSV, (uint32_t)i, stream, s, a.key0, a.key1, zz);
+#pragma unroll
+    for (int v = 0; v < NSV; ++v) xi[v] = ((R)a.sqrt_dt * (R)zz[v]) * gf[v];
+  } else {
+#pragma unroll
+    for (int v = 0; v < NSV; ++v) xi[v] = R(0);
+  }
+  if constexpr (EXT) {
+    if (a.stim_tp) {   // simulator.py:300-306: spatial pattern x temporal value of this step
+      const R sv = (R)(a.stim_sp[i] * a.stim_tp[k]);
+      R st[NSV];
+#pragma unroll
+      for (int v = 0; v < NSV; ++v) st[v] = ((a.stim_mask >> v) & 1u) ? sv : R(0);
+      scheme_step<R, M, true>(a.scheme, x, c, pp, dt, xi, st);
+    } else {
+      scheme_step<R, M>(a.scheme, x, c, pp, dt, xi);
+    }
+  } else {
+    scheme_step<R, M>(a.scheme, x, c, pp, dt, xi);
+  }
+#pragma unroll
+  for (int v = 0; v < NSV; ++v) {
+    bad |= !isfinite(x[v]);
+    s_x[(i * NSV + v) * T + t] = x[v];
+  }
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) sink(ch, C::template channel<R>(x, ch, cp));
+
+  monitors_node<R, M, CF, EXT>(a, kc, smem, i, t, T, N, live, k, buf, x, c, false);
+}
+
 // RH: history / weight / coupling-sum type, R: state, integration and
 // monitor type (float/float, double/double, or float/double -- the
 // reference's own mix).  Float64 histories get up to 8 waves per workgroup
@@ -359,10 +393,11 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     for (int m = 0; m < TVBK_KMAX_MON; ++m) {
       MonArgs mo = a.mon[m];
       if (m < a.n_mon) {
-        mo.r0 = (int)(td_step0 % mo.istep);
-        mo.qe0 = td_step0 / mo.istep;
-        mo.rb0 = (int)(td_step0 % mo.ii);
-        mo.qb0 = td_step0 / mo.ii;
+        const int64_t p0 = td_step0 - (mo.src ? 1 : 0);   // step of the call's first sample
+        mo.r0 = (int)(p0 % mo.istep);
+        mo.qe0 = p0 / mo.istep;
+        mo.rb0 = (int)(p0 % mo.ii);
+        mo.qb0 = p0 / mo.ii;
         step_info(mo, 0, kc.si[0][m]);
       }
       kc.mon[m] = mo;
@@ -632,7 +667,8 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(i * NCT + v) * T + t], cpv);
     RH* wr = ring + (wslot * N + i) * rstride + t;
     RH* st = cbuf + (i * NCT) * T + t;
-    update_node<R, M, CF, RH, UM>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s_, buf, par, pst, c, bad,
+    update_node<R, M, CF, RH, UM, PIPE == 0>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s_, buf, par,
+                                             pst, c, bad,
                                   [&](int ch, R v) {
                                     if (stage) st[ch * T] = (RH)v;
                                     else wr[ch * T] = (RH)v;
@@ -693,7 +729,12 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
       __syncthreads();
     }
   } else {
-  for (int k = 0; k < nsteps; ++k, ++s) {
+  // afferent monitors sample, at step s, the coupling step s + 1 integrates
+  // with (simulator.py:311-316): the call's last sample needs the coupling of
+  // the step after it, gathered by one more phase A (the next call gathers it
+  // again) followed by those monitors only
+  const int kend = nsteps + (a.has_aff ? 1 : 0);
+  for (int k = 0; k < kend; ++k, ++s) {
     const int buf = k & 1;
     RH* cbuf = s_c + (int64_t)buf * N * NCT * T;
     // ================= phase A: coupling of step s ========================
@@ -701,6 +742,20 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     else if (a.csr_in_lds) phase_a(rec_lds, l_rp, cbuf);
     else phase_a(rec_glb, g_rp, cbuf);
     __syncthreads();
+    if (k == nsteps) {
+      for (int i = tid / T; i < N; i += nthr / T) {
+        R c[NCT], x[NSV];
+        R cpv[NCP > 0 ? NCP : 1];
+#pragma unroll
+        for (int v = 0; v < NCP; ++v) cpv[v] = lane_param<R>(a, par, pst, NDP + v);
+#pragma unroll
+        for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(i * NCT + v) * T + t], cpv);
+#pragma unroll
+        for (int v = 0; v < NSV; ++v) x[v] = s_x[(i * NSV + v) * T + t];
+        monitors_node<R, M, CF, true>(a, kc, smem, i, t, T, N, live, nsteps, buf, x, c, true);
+      }
+      break;
+    }
     // ================= phase B: integrate, write history, monitors =========
     const int wslot = (smod + 1 == H) ? 0 : smod + 1;   // s % H
     for (int i = tid / T; i < N; i += nthr / T) b_item(i, cbuf, k, s, buf, wslot, false);
@@ -895,7 +950,8 @@ __global__ void k_init_uniform(InitArgs a) {
 // ---------------------------------------------------------------------------
 template <class RH, class R, class M, int CF>
 hipError_t launch_step_t(const StepArgs& a, int ntiles, int nthreads, size_t lds, hipStream_t st) {
-  const bool pipe = a.uniform_rows && !a.zero_delay && Cfun<M, CF>::NCH * a.T <= 64;
+  // stimulus and afferent monitors run in the sequential loop only
+  const bool pipe = a.uniform_rows && !a.zero_delay && Cfun<M, CF>::NCH * a.T <= 64 && !a.stim_tp && !a.has_aff;
   const uint64_t mmask = M::NDP >= 64 ? ~0ull : ((1ull << M::NDP) - 1);
   const bool um = pipe && (a.par_umask & mmask) == mmask && !a.spatial;
   auto kern = !pipe ? k_step<RH, R, M, CF, 0, false> : um ? k_step<RH, R, M, CF, 1, true> : k_step<RH, R, M, CF, 1, false>;

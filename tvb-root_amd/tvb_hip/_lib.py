@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtvbk.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 MAX_OBS = 8
 MAX_MON = 4
 
@@ -24,6 +24,7 @@ EULER_DET, EULER_STO, HEUN_DET, HEUN_STO, IDENTITY = 0, 1, 2, 3, 4
 F32, F64, MIXED = 0, 1, 2   # MIXED: float32 history, weights and sums; float64 state
 NOISE_NONE, NOISE_INJECTED, NOISE_PHILOX = 0, 1, 2
 MON_RAW, MON_TAVG, MON_BOLD = 0, 1, 2
+MON_SUBSAMPLE, MON_GAVG, MON_SAVG, MON_AFF, MON_AFF_TAVG = 3, 4, 5, 6, 7
 KERNEL_AUTO, KERNEL_BATCH_LANE, KERNEL_EDGE_LANE, KERNEL_DENSE_MFMA = 0, 1, 2, 3
 
 
@@ -47,7 +48,8 @@ class Monitor(C.Structure):
     _fields_ = [("kind", C.c_int32), ("istep", C.c_int32), ("n_voi", C.c_int32),
                 ("voi", C.c_int32 * MAX_OBS), ("interim_istep", C.c_int32),
                 ("n_taps", C.c_int32), ("taps", C.c_void_p), ("k1_v0", C.c_double),
-                ("subtract_one", C.c_int32), ("keep_stock", C.c_int32)]
+                ("subtract_one", C.c_int32), ("keep_stock", C.c_int32),
+                ("spatial_mask", C.c_void_p), ("n_area", C.c_int32)]
 
 
 class PlanDesc(C.Structure):
@@ -81,6 +83,7 @@ def _bind(lib):
         "tvbk_set_noise": ([P, P, C.c_int64], C.c_int),
         "tvbk_run_steps": ([P, C.c_int64], C.c_int),
         "tvbk_set_steps": ([P, P], C.c_int),
+        "tvbk_set_stimulus": ([P, P, P, C.c_int64, C.c_uint32], C.c_int),
         "tvbk_sync": ([P], C.c_int),
         "tvbk_monitor_layout": ([P, C.c_int32, P, P, P, P, P], C.c_int),
         "tvbk_read_monitor": ([P, C.c_int32, P, C.c_int64], C.c_int),
@@ -107,7 +110,7 @@ EXPORTED = (
     "tvbk_abi_version", "tvbk_model_info", "tvbk_plan_create", "tvbk_plan_destroy",
     "tvbk_plan_footprint", "tvbk_poll", "tvbk_stream_create", "tvbk_stream_destroy",
     "tvbk_set_history", "tvbk_init_uniform", "tvbk_set_noise", "tvbk_run_steps",
-    "tvbk_set_steps", "tvbk_sync", "tvbk_monitor_layout", "tvbk_read_monitor",
+    "tvbk_set_steps", "tvbk_set_stimulus", "tvbk_sync", "tvbk_monitor_layout", "tvbk_read_monitor",
     "tvbk_monitor_device_ptr", "tvbk_export_state", "tvbk_export_monitor", "tvbk_import_monitor",
     "tvbk_read_flags",
     "tvbk_debug_slots", "tvbk_plan_shape", "tvbk_last_run_ms", "tvbk_balloon", "tvbk_last_error")

@@ -28,7 +28,10 @@ CFUNS = {"Linear": (L.CFUN_LINEAR, ("a", "b")),
          "SigmoidalJansenRit": (L.CFUN_SIGMOID_JR, ("cmin", "cmax", "midpoint", "r", "a"))}
 SCHEMES = {"HeunDeterministic": L.HEUN_DET, "HeunStochastic": L.HEUN_STO,
            "EulerDeterministic": L.EULER_DET, "EulerStochastic": L.EULER_STO}
-MONITORS = {"Raw": L.MON_RAW, "TemporalAverage": L.MON_TAVG, "Bold": L.MON_BOLD}
+MONITORS = {"Raw": L.MON_RAW, "TemporalAverage": L.MON_TAVG, "Bold": L.MON_BOLD,
+            "RawVoi": L.MON_SUBSAMPLE, "SubSample": L.MON_SUBSAMPLE, "GlobalAverage": L.MON_GAVG,
+            "SpatialAverage": L.MON_SAVG, "AfferentCoupling": L.MON_AFF,
+            "AfferentCouplingTemporalAverage": L.MON_AFF_TAVG}
 PRECISIONS = {"fp32": L.F32, "float32": L.F32, "fp64": L.F64, "float64": L.F64, "mixed": L.MIXED}
 
 
@@ -78,7 +81,11 @@ def monitor_spec(mon, dt, keep_stock=False):
         return MonitorSpec(kind, mon.istep, voi, interim_istep=mon._interim_istep, taps=taps,
                            k1_v0=k1v0, subtract_one=sub, period=mon.period, name="Bold",
                            keep_stock=keep_stock)
-    return MonitorSpec(kind, mon.istep, voi, period=getattr(mon, "period", None), name=_name(mon))
+    mask = None
+    if kind == L.MON_SAVG:
+        mask = np.asarray(mon.spatial_mask, np.int32)
+    return MonitorSpec(kind, mon.istep, voi, period=getattr(mon, "period", None), name=_name(mon),
+                       spatial_mask=mask)
 
 
 def _param_vector(sim, model_id, cfun_params, stochastic, N):
@@ -130,7 +137,11 @@ class HipBackend:
             if not cond:
                 raise NotImplementedError("Unsupported simulator component. " + what)
         need(getattr(sim, "surface", None) is None, "Surface simulation not supported.")
-        need(getattr(sim, "stimulus", None) is None, "Stimulus not supported.")
+        stim = getattr(sim, "stimulus", None)
+        if stim is not None:
+            sp = getattr(stim, "spatial_pattern", None)
+            need(sp is not None and np.size(sp) == sim.connectivity.number_of_regions,
+                 "Only region stimuli (StimuliRegion with a configured spatial pattern) are supported.")
         need(_name(sim.model) in MODELS, "Given: %s\nExpected one of: %s" % (_name(sim.model), tuple(MODELS)))
         need(getattr(sim.model, "number_of_modes", 1) == 1, "Multi-mode models not supported.")
         need(_name(sim.coupling) in CFUNS, "Given: %s\nExpected one of: %s" % (_name(sim.coupling), tuple(CFUNS)))
@@ -169,7 +180,7 @@ class HipBackend:
         for m, mon in enumerate(self._monitors(sim)):
             stock = getattr(mon, "_stock", None)
             interim = getattr(mon, "_interim_stock", None)
-            if stock is None or plan.monitors[m].kind == L.MON_RAW:
+            if stock is None or not plan.monitors[m].stateful:
                 continue
             if not np.any(stock) and (interim is None or not np.any(interim)):
                 continue
@@ -179,7 +190,7 @@ class HipBackend:
         """The plan's monitor state back into the simulator's monitor stocks."""
         for m, mon in enumerate(self._monitors(sim)):
             ms = plan.monitors[m]
-            if ms.kind == L.MON_RAW:
+            if not ms.stateful:
                 continue
             if ms.kind == L.MON_BOLD and not ms.keep_stock:
                 continue
@@ -245,6 +256,32 @@ class HipBackend:
         self._import_monitors(plan, 0, sim)
         return plan
 
+    @staticmethod
+    def _stimulus(sim, simulation_length, nstep):
+        """(spatial (N,), temporal (>= nstep,), state-variable mask) of the
+        simulator's region stimulus for one run, or None.  The temporal
+        pattern is sampled on the run's own time grid and indexed by the
+        run's step (Simulator._prepare_stimulus / _loop_update_stimulus,
+        simulator.py:290-306)."""
+        stim = getattr(sim, "stimulus", None)
+        if stim is None:
+            return None
+        length = float(simulation_length if simulation_length is not None else sim.simulation_length)
+        time = np.r_[0.0:length:sim.integrator.dt]
+        stim.configure_time(time.reshape((1, -1)))
+        tp = np.asarray(stim.temporal_pattern, np.float64).reshape(-1)
+        sp = np.asarray(stim.spatial_pattern, np.float64).reshape(-1)
+        if tp.size < nstep:
+            raise ValueError("the stimulus time grid (%d samples of simulation_length %g ms) is shorter than the "
+                             "run (%d steps)" % (tp.size, length, nstep))
+        stvar = getattr(sim.model, "stvar", None)
+        if stvar is None:
+            stvar = sim.model.cvar
+        mask = 0
+        for v in np.atleast_1d(stvar):
+            mask |= 1 << int(v)
+        return sp, tp, mask
+
     # --------------------------------------------------------------- running
     def run_sim(self, sim, nstep=None, simulation_length=None, precision="auto", noise="legacy",
                 chunk=65536, kernel="auto", write_back=True, **_ignored):
@@ -269,9 +306,12 @@ class HipBackend:
             stochastic = plan.noise == L.NOISE_INJECTED
             ts = [[] for _ in plan.monitors]
             xs = [[] for _ in plan.monitors]
+            stim = self._stimulus(sim, simulation_length, nstep)
             done = 0
             while done < nstep:
                 n = min(chunk, nstep - done)
+                if stim is not None:
+                    plan.set_stimulus(stim[0], stim[1][done:done + n], stim[2])
                 if stochastic:
                     if isinstance(noise, np.ndarray):
                         z = noise[done:done + n]

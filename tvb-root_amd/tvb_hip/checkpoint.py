@@ -126,7 +126,7 @@ class SimulationHistory:
         names = list(monitor_names) if monitor_names else [ms.name or str(ms.kind) for ms in plan.monitors]
         h.monitor_names = names
         for m, ms in enumerate(plan.monitors):
-            if ms.kind == L.MON_RAW:
+            if not ms.stateful:
                 continue
             if ms.kind == L.MON_BOLD and not ms.keep_stock:
                 raise ValueError("monitor %d: a Bold stock is only exported by a plan built with keep_stock" % m)
@@ -160,7 +160,7 @@ class SimulationHistory:
         plan.set_history(inst, hist, state, int(self.current_step))
         for m, ms in enumerate(plan.monitors):
             stock = getattr(self, "monitor_stock_%d" % (m + 1))
-            if ms.kind == L.MON_RAW or stock is None:
+            if not ms.stateful or stock is None:
                 continue
             plan.import_monitor(inst, m, stock, self.monitor_interim.get(m + 1))
         return plan

@@ -42,7 +42,7 @@ class MonitorSpec:
     """Device monitor description (``tvbk_monitor``)."""
 
     def __init__(self, kind, istep, voi, interim_istep=0, taps=None, k1_v0=0.0,
-                 subtract_one=True, period=None, name=None, keep_stock=False):
+                 subtract_one=True, period=None, name=None, keep_stock=False, spatial_mask=None):
         self.kind = kind
         self.istep = int(istep)
         self.voi = [int(v) for v in voi]
@@ -53,10 +53,28 @@ class MonitorSpec:
         self.period = period
         self.name = name
         self.keep_stock = bool(keep_stock)
+        # SpatialAverage: area of every node; GlobalAverage: one area
+        self.spatial_mask = None if spatial_mask is None else np.ascontiguousarray(spatial_mask, np.int32)
+        if kind == L.MON_SAVG:
+            self.n_area = int(self.spatial_mask.max()) + 1
+        elif kind == L.MON_GAVG:
+            self.n_area = 1
+        else:
+            self.n_area = 0
+
+    @property
+    def averaging(self):
+        """TemporalAverage-like: a window of istep samples (and its stock)."""
+        return self.kind in (L.MON_TAVG, L.MON_AFF_TAVG)
+
+    @property
+    def stateful(self):
+        """Keeps state between steps (a stock in the reference)."""
+        return self.averaging or self.kind == L.MON_BOLD
 
     def times(self, first_step, count, dt):
         steps = first_step + self.istep * np.arange(count)
-        if self.kind == L.MON_TAVG:
+        if self.averaging:
             return (steps - self.istep / 2.0) * dt      # monitors.py:389-391
         return steps * dt                                # monitors.py:188-190, :953
 
@@ -110,6 +128,9 @@ class Plan:
                 mo.k1_v0 = ms.k1_v0
                 mo.subtract_one = int(ms.subtract_one)
                 mo.keep_stock = int(ms.keep_stock)
+            if ms.kind == L.MON_SAVG:
+                mo.spatial_mask = L.ptr(ms.spatial_mask)
+                mo.n_area = ms.n_area
         d.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
         d.inst_stream = L.ptr(self.inst_stream)
         d.max_steps = self.max_steps
@@ -180,6 +201,17 @@ class Plan:
         self._z_keep = z
         self._chk(self._lib.tvbk_set_noise(self.handle, L.ptr(z), z.shape[0]))
 
+    def set_stimulus(self, spatial, temporal, svar_mask):
+        """Region stimulus of the next run: spatial (N,) x temporal[k] at the
+        run's k-th step for the state variables in svar_mask (bits);
+        temporal None removes it."""
+        if temporal is None:
+            self._chk(self._lib.tvbk_set_stimulus(self.handle, None, None, 0, 0))
+            return
+        sp = np.ascontiguousarray(np.asarray(spatial, np.float64).reshape(-1))
+        tp = np.ascontiguousarray(np.asarray(temporal, np.float64).reshape(-1))
+        self._chk(self._lib.tvbk_set_stimulus(self.handle, L.ptr(sp), L.ptr(tp), tp.size, int(svar_mask)))
+
     def set_steps(self, remaining):
         if remaining is None:
             self._chk(self._lib.tvbk_set_steps(self.handle, None))
@@ -230,13 +262,13 @@ class Plan:
         insts = range(B) if instances is None else instances
         out = []
         if lay == 0:
-            N = self.conns[0].n_node
+            N = ms.n_area or self.conns[0].n_node
             data = buf.reshape((int(cnt[0]), nv, N, B))
             for b in insts:
                 out.append((ms.times(first[b], cnt[b], self.dt), data[..., b]))
         else:
             for b in insts:
-                N = self.conns[0 if self.inst_conn is None else self.inst_conn[b]].n_node
+                N = ms.n_area or self.conns[0 if self.inst_conn is None else self.inst_conn[b]].n_node
                 n = int(cnt[b]) * nv * N
                 out.append((ms.times(first[b], cnt[b], self.dt),
                             buf[off[b]:off[b] + n].reshape((int(cnt[b]), nv, N))))
@@ -270,10 +302,10 @@ class Plan:
         (monitor built with keep_stock) and ``_interim_stock``
         (interim_istep, nvoi, N, 1); Raw -> (None, None)."""
         ms = self.monitors[m]
-        if ms.kind == L.MON_RAW:
+        if not ms.stateful:
             return None, None
         N, nv = self._n_node(inst), len(ms.voi)
-        if ms.kind == L.MON_TAVG:
+        if ms.averaging:
             st, it = np.empty((ms.istep, nv, N)), None
         else:
             st = np.empty((len(ms.taps), nv, N)) if stock else None
@@ -287,10 +319,10 @@ class Plan:
         monitor has after SimulationHistory.fill_into.  inst -1 = every
         instance."""
         ms = self.monitors[m]
-        if ms.kind == L.MON_RAW:
+        if not ms.stateful:
             return
         N, nv = self._n_node(inst), len(ms.voi)
-        rows = ms.istep if ms.kind == L.MON_TAVG else len(ms.taps)
+        rows = ms.istep if ms.averaging else len(ms.taps)
         st = np.ascontiguousarray(np.asarray(stock, np.float64).reshape((rows, nv, N)))
         it = None
         if interim is not None and ms.kind == L.MON_BOLD:

@@ -52,7 +52,8 @@ class Connectivity(_Traited):
     """Connectivity (datatypes/connectivity.py:45): weights[i, j] is the
     connection from j to i; delays = tract_lengths / speed (:275);
     idelays = rint(delays / dt) (:321); horizon = max(idelays) + 1 (:323)."""
-    _defaults = dict(weights=None, tract_lengths=None, speed=np.array([3.0]), centres=None, region_labels=None)
+    _defaults = dict(weights=None, tract_lengths=None, speed=np.array([3.0]), centres=None, region_labels=None,
+                     cortical=None, hemispheres=None)
     _extra = ("number_of_regions",)
 
     def configure(self):
@@ -91,6 +92,68 @@ class FirstOrderVolterra(_Traited):
         return 1 / 3. * np.exp(-0.5 * (var / p["tau_s"])) * (np.sin(q * var)) / q
 
 
+class DiscreteEquation(_Traited):
+    """equations.py:136-147: the value itself (a per-region weight)."""
+    _defaults = dict(parameters=None)
+
+    def evaluate(self, var):
+        return var
+
+
+class PulseTrain(_Traited):
+    """equations.py:340-364: amp where var > onset and (var - onset) % T < tau."""
+    _defaults = dict(parameters=None)
+
+    def __init__(self, **kw):
+        super().__init__(**kw)
+        if self.parameters is None:
+            self.parameters = {"T": 42.0, "tau": 13.0, "amp": 1.0, "onset": 30.0}
+
+    def evaluate(self, var):
+        p = self.parameters
+        return np.where((var > p["onset"]) & (((var - p["onset"]) % p["T"]) < p["tau"]), p["amp"], 0)
+
+
+class StimuliRegion(_Traited):
+    """patterns.py:40-185 for regions: spatial pattern = spatial.evaluate(weight)
+    per node, temporal pattern = temporal.evaluate(time); at a step index k the
+    stimulus is spatial_pattern * temporal_pattern[0, k]."""
+    _defaults = dict(temporal=None, spatial=None, connectivity=None, weight=None)
+    _scalar = ("temporal", "spatial", "connectivity", "weight")
+
+    def __init__(self, **kw):
+        super().__init__(**kw)
+        if self.spatial is None:
+            self.spatial = DiscreteEquation()
+        if self.temporal is None:
+            self.temporal = PulseTrain()
+        self._spatial_pattern = None
+        self._temporal_pattern = None
+
+    def configure(self):
+        return self
+
+    def configure_space(self, region_mapping=None):
+        w = np.asarray(self.weight, np.float64)[:, np.newaxis]
+        self._spatial_pattern = np.sum(self.spatial.evaluate(w), axis=1)[:, np.newaxis]
+
+    def configure_time(self, time):
+        self._temporal_pattern = np.reshape(self.temporal.evaluate(time), (1, -1))
+
+    @property
+    def spatial_pattern(self):
+        return self._spatial_pattern
+
+    @property
+    def temporal_pattern(self):
+        return self._temporal_pattern
+
+    def __call__(self, temporal_indices=None):
+        if temporal_indices is None:
+            return self._spatial_pattern * self._temporal_pattern
+        return self._spatial_pattern * self._temporal_pattern[0, temporal_indices]
+
+
 # ---------------------------------------------------------------------------
 # models
 # ---------------------------------------------------------------------------
@@ -98,6 +161,7 @@ class FirstOrderVolterra(_Traited):
 class Model(_Traited):
     state_variables = ()
     cvar = None
+    stvar = None          # stimulated variables; the coupling variables unless given (models/base.py:91-92)
     number_of_modes = 1
     parameter_names = ()
     state_variable_boundaries = None
@@ -120,6 +184,8 @@ class Model(_Traited):
 
     def configure(self):
         self.state_variable_mask = np.ones(self.nvar, bool)
+        if self.stvar is None:
+            self.stvar = np.array(self.cvar, np.int32).copy()
         return self
 
     def initial(self, dt, history_shape, rng=np.random):
@@ -155,6 +221,7 @@ class MontbrioPazoRoxin(Model):
     parameter_names = ("tau", "I", "Delta", "J", "eta", "Gamma", "cr", "cv")
     state_variables = ("r", "V")
     cvar = np.array([0, 1], np.int32)
+    stvar = np.array([1], np.int32)          # infinite_theta.py:169
     _voi_default = ("r", "V")
     _svr_default = {"r": np.array([0.0, 2.0]), "V": np.array([-2.0, 1.5])}
 
@@ -326,6 +393,79 @@ class TemporalAverage(Monitor):
         self._stock = np.zeros((self.istep, self.voi.shape[0], sim.number_of_nodes, 1))
 
 
+class RawVoi(Raw):
+    """monitors.py:193-211: Raw restricted to the chosen variables of interest."""
+
+    def _config_vois(self, sim):
+        Monitor._config_vois(self, sim)
+
+
+class SubSample(Monitor):
+    """monitors.py:205-214: the observed state every istep steps."""
+
+
+class GlobalAverage(Monitor):
+    """monitors.py:344-360: mean over all nodes every istep steps."""
+
+
+class SpatialAverage(Monitor):
+    """monitors.py:217-342: spatial_mean . state every istep steps, over the
+    areas of ``spatial_mask`` (default: the connectivity's cortical or
+    hemispheres vector)."""
+    _defaults = dict(period=0.9765625, variables_of_interest=None, spatial_mask=None, default_mask="cortical")
+    _scalar = ("period", "variables_of_interest", "spatial_mask", "default_mask")
+
+    @staticmethod
+    def _support_bool_mask(mask):
+        m = np.array([int(v) for v in mask])
+        u = np.unique(m)
+        if len(u) == 1 and u[0] == 1:
+            return np.zeros(len(m), dtype=np.int_)
+        return m
+
+    def config_for_sim(self, sim):
+        super().config_for_sim(sim)
+        if self.spatial_mask is None:
+            conn = sim.connectivity
+            src = conn.cortical if self.default_mask == "cortical" else conn.hemispheres
+            if src is None:
+                raise ValueError("SpatialAverage needs a spatial_mask (or the connectivity's %s vector)"
+                                 % self.default_mask)
+            self.spatial_mask = self._support_bool_mask(src)
+        self.spatial_mask = np.asarray(self.spatial_mask, int)
+        if self.spatial_mask.size != sim.number_of_nodes:
+            raise ValueError("spatial_mask must be a vector of length number_of_nodes.")
+        areas = np.unique(self.spatial_mask)
+        if not np.all(areas == np.arange(len(areas))):
+            raise ValueError("Areas in the spatial_mask must be specified as a contiguous set of indices starting "
+                             "from zero.")
+        ss = np.zeros((sim.number_of_nodes, len(areas)))
+        ss[np.arange(sim.number_of_nodes), self.spatial_mask] = 1
+        ss = ss.T
+        self.spatial_mean = ss / np.sum(ss, axis=1)[:, np.newaxis]
+
+
+class AfferentCoupling(RawVoi):
+    """monitors.py:392-410: the node coupling of every step; voi index the
+    coupling variables."""
+    _defaults = dict(period=0.0, variables_of_interest=None)
+
+    def _config_vois(self, sim):
+        self.voi = self.variables_of_interest
+        if self.voi is None or np.size(self.voi) == 0:
+            self.voi = np.r_[:len(sim.model.cvar)]
+        self.voi = np.asarray(self.voi, int)
+
+
+class AfferentCouplingTemporalAverage(AfferentCoupling):
+    """monitors.py:413-431."""
+    _defaults = dict(period=0.9765625, variables_of_interest=None)
+
+    def _config_time(self, sim):
+        Monitor._config_time(self, sim)
+        self._stock = np.zeros((self.istep, self.voi.shape[0], sim.number_of_nodes, 1))
+
+
 class Bold(Monitor):
     """monitors.py:834-962 (FIR of 4 ms means with the FirstOrderVolterra HRF)."""
     _defaults = dict(period=2000.0, variables_of_interest=None, hrf_kernel=None, hrf_length=20000.0)
@@ -411,6 +551,9 @@ class Simulator(_Traited):
         if isinstance(self.integrator, IntegratorStochastic):
             self._configure_integrator_noise()
         self._configure_history()
+        if self.stimulus is not None:         # simulator.py:209-210, :437-443
+            self.stimulus.configure()
+            self.stimulus.configure_space()
         for m in self.monitors:
             m.config_for_sim(self)
         return self

@@ -31,7 +31,7 @@ def initial_history(model, H, n, seed, spread=1.0):
 def make_case(model, n=16, density=0.3, lengths=(10.0, 60.0), speed=3.0, dt=0.1,
               scheme="HeunDeterministic", coupling="Linear", cparams=None, mparams=None,
               nsig=None, seed=0, monitors=(("Raw", None, None),), spread=1.0, weights=None,
-              tract_lengths=None, stimulus=None):
+              tract_lengths=None, stimulus=None, initial_conditions=None):
     """``stimulus``: (weights (N,), temporal equation with ``evaluate(t)``,
     simulation_length ms) for a region stimulus in both expressions."""
     if weights is None:
@@ -40,7 +40,7 @@ def make_case(model, n=16, density=0.3, lengths=(10.0, 60.0), speed=3.0, dt=0.1,
         w, L = np.asarray(weights, float), np.asarray(tract_lengths, float)
         n = w.shape[0]
     _, _, H = set_idelays(L, speed, dt)
-    ic = initial_history(model, H, n, seed + 1000, spread)
+    ic = initial_history(model, H, n, seed + 1000, spread) if initial_conditions is None else initial_conditions
     cparams = dict(cparams or {})
     mparams = dict(mparams or {})
     kw = dict(coupling=coupling, coupling_params=cparams, model_params=mparams,

@@ -402,33 +402,55 @@ class HipBackend:
         return plan
 
     def run_batch(self, sims, nstep, precision="auto", noise="inject", steps=None, kernel="auto", tile=0,
-                  seed=0, streams=None):
-        """Run several simulators in one launch.  ``nstep``: steps of the
-        longest; ``steps``: per-simulator step counts (uneven durations).
-        ``noise``: "philox", or an array per simulator of standard normals
-        (n, nsvar, N) to inject.  Returns one ``[(t, x), ...]`` per simulator."""
+                  seed=0, streams=None, chunk=65536):
+        """Run several simulators in one launch per chunk of steps.  ``nstep``:
+        steps of the longest; ``steps``: per-simulator step counts (uneven
+        durations).  ``noise``: "inject" draws each simulator's normals from
+        its own RandomState in the reference's order, "philox" uses the device
+        generator, or a list with one array (n, nsvar, N) per simulator.
+        Returns one ``[(t, x), ...]`` per simulator."""
         sims = list(sims)
+        nstep = int(nstep)
+        chunk = max(1, min(int(chunk), nstep))
         inject = not (isinstance(noise, str) and noise == "philox")
         plan = self.build_batch_plan(sims, precision=precision, noise="inject" if inject else "philox",
-                                     max_steps=nstep, kernel=kernel, tile=tile, seed=seed, streams=streams)
+                                     max_steps=chunk, kernel=kernel, tile=tile, seed=seed, streams=streams)
         try:
+            todo = np.full(len(sims), nstep, np.int64) if steps is None else np.asarray(steps, np.int64)
             if steps is not None:
-                plan.set_steps(np.asarray(steps, np.int64))
-            if plan.noise == L.NOISE_INJECTED:
-                nmax = max(c.n_node for c in plan.conns)
-                nsv = len(sims[0].model.state_variables)
-                z = np.zeros((nstep, nsv, nmax, len(sims)))
-                for b, s in enumerate(sims):
-                    zb = noise[b] if not isinstance(noise, str) else \
-                        s.integrator.noise.random_stream.normal(size=(nstep, nsv, s.connectivity.number_of_regions))
-                    zb = np.asarray(zb, np.float64)
-                    z[:zb.shape[0], :, :zb.shape[2], b] = zb
-                plan.set_noise(z)
-            plan.run(nstep)
-            out = [[] for _ in sims]
-            for m in range(len(plan.monitors)):
-                for b, (t, x) in enumerate(plan.read(m)):
-                    out[b].append((t, np.asarray(x, np.float64)[..., np.newaxis]))
+                plan.set_steps(todo)
+            nmax = max(c.n_node for c in plan.conns)
+            nsv = len(sims[0].model.state_variables)
+            parts = [[[] for _ in plan.monitors] for _ in sims]
+            done = 0
+            while done < nstep:
+                n = min(chunk, nstep - done)
+                if plan.noise == L.NOISE_INJECTED:
+                    z = np.zeros((n, nsv, nmax, len(sims)))
+                    for b, s in enumerate(sims):
+                        nb = int(min(n, max(0, todo[b] - done)))   # steps this simulator still takes
+                        if nb == 0:
+                            continue
+                        N = s.connectivity.number_of_regions
+                        if isinstance(noise, str):
+                            zb = s.integrator.noise.random_stream.normal(size=(nb, nsv, N))
+                        else:
+                            zb = np.asarray(noise[b][done:done + nb], np.float64)
+                        z[:nb, :, :N, b] = zb
+                    plan.set_noise(z)
+                plan.run(n)
+                for m in range(len(plan.monitors)):
+                    for b, (t, x) in enumerate(plan.read(m)):
+                        parts[b][m].append((t, np.asarray(x, np.float64)))
+                done += n
+            out = []
+            for b in range(len(sims)):
+                res = []
+                for m in range(len(plan.monitors)):
+                    t = np.concatenate([p[0] for p in parts[b][m]])
+                    x = np.concatenate([p[1] for p in parts[b][m]], axis=0)
+                    res.append((t, x[..., np.newaxis]))
+                out.append(res)
             self.last_flags = plan.flags()
             self.last_plan_shape = (plan.kernel, plan.tile, plan.waves, plan.lds_bytes)
         finally:

@@ -146,6 +146,10 @@ struct tvbk_plan {
   int ukind[TVBK_MAX_MON] = {0}, src[TVBK_MAX_MON] = {0}, n_area[TVBK_MAX_MON] = {0};
   DevBuf d_raw[TVBK_MAX_MON], d_area_of[TVBK_MAX_MON], d_area_w[TVBK_MAX_MON], d_area_jobs[TVBK_MAX_MON];
   std::vector<AreaJob> jobs_host[TVBK_MAX_MON];
+  // edge-lane instances split over several tiles: block length (steps,
+  // 0 = no split) and the staged history of a block
+  int split_k = 0;
+  DevBuf d_stage;
   // region stimulus of the next run
   DevBuf d_stim_sp, d_stim_tp;
   const double* stim_sp = nullptr;
@@ -248,6 +252,7 @@ static void destroy(tvbk_plan* p) {
   }
   dev_free(&p->d_stim_sp);
   dev_free(&p->d_stim_tp);
+  dev_free(&p->d_stage);
   dev_free(&p->d_dense);
   if (p->ev0) hipEventDestroy(p->ev0);
   if (p->ev1) hipEventDestroy(p->ev1);
@@ -571,6 +576,9 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
     td.n_inst = (T == 1) ? 1 : std::min(T, B - b0);
     td.conn = inst_conn[b0];
     const ConnDesc& cd = p->conns[td.conn];
+    td.node0 = 0;
+    td.node1 = cd.n_node;
+    td.parts = 1;
     for (int l = 0; l < td.n_inst; ++l) {
       p->tile_of[b0 + l] = k;
       p->lane_of[b0 + l] = l;
@@ -596,6 +604,85 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
           bst_len[m] += (int64_t)d.mon[m].n_taps * d.mon[m].n_voi * cd.n_node * T;
         }
       }
+  }
+
+  // ---- edge-lane kernel: split heavy instances over several tiles (node
+  // ranges with balanced edge counts), so that one large connectome does not
+  // hold one CU for the whole call while the others idle.  Split tiles step
+  // in blocks of K <= d_min steps: a block reads only history written before
+  // it, each tile stages its nodes' history writes, and a commit between
+  // blocks stores them in the ring.  TVBK_SPLIT: unset = automatic, 0 = off,
+  // n > 1 = n parts for every instance that can be split.
+  int64_t stage_len = 0;
+  if (T == 1 && kernel == TVBK_KERNEL_EDGE_LANE && B > 0) {
+    const char* env = std::getenv("TVBK_SPLIT");
+    const int forced = env ? std::atoi(env) : -1;
+    int n_cu = 256;
+    hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+    std::vector<int64_t> cost(B);
+    int64_t total = 0;
+    for (int b = 0; b < B; ++b) {
+      const ConnDesc& cd = p->conns[inst_conn[b]];
+      cost[b] = cd.nnz + cd.n_node;
+      total += cost[b];
+    }
+    const int64_t target = std::max<int64_t>(total / (2 * (int64_t)std::max(n_cu, 1)), 4096);
+    int kmin = 1 << 30;
+    std::vector<TileDesc> extra;
+    for (int k = 0; k < ntiles && forced != 0; ++k) {
+      TileDesc& td = p->tiles[k];
+      const ConnDesc& cd = p->conns[td.conn];
+      const tvbk_connectome& cn = d.conns[td.conn];
+      if (cd.d_min < 4) continue;                      // blocks of at least four steps
+      int parts = forced > 1 ? forced : (int)((cost[td.inst0] + target - 1) / target);
+      parts = std::min(parts, std::min(32, cd.n_node / 16));
+      if (parts < 2) continue;
+      // node boundaries at equal shares of sum(row length + 1)
+      std::vector<int> bound(parts + 1, cd.n_node);
+      bound[0] = 0;
+      int64_t acc = 0;
+      const int64_t tot = cd.nnz + cd.n_node;
+      int j = 1;
+      for (int i = 0; i < cd.n_node && j < parts; ++i) {
+        acc += (cn.row_ptr[i + 1] - cn.row_ptr[i]) + 1;
+        if (acc * parts >= tot * j) bound[j++] = i + 1;
+      }
+      for (; j < parts; ++j) bound[j] = cd.n_node;
+      td.parts = parts;
+      td.node1 = bound[1];
+      for (int q = 1; q < parts; ++q) {
+        TileDesc t2 = td;
+        t2.node0 = bound[q];
+        t2.node1 = bound[q + 1];
+        if (t2.node1 > t2.node0) extra.push_back(t2);
+      }
+      kmin = std::min(kmin, cd.d_min);
+    }
+    if (!extra.empty()) {
+      p->split_k = std::min(kmin, 4096);
+      // staging sized for K steps of each split instance (K <= its d_min)
+      stage_len = 0;
+      for (TileDesc& td : p->tiles)
+        if (td.parts > 1) {
+          td.stage_off = stage_len;
+          stage_len += (int64_t)p->split_k * p->conns[td.conn].n_node * p->nch;
+        }
+      for (TileDesc& t2 : extra) t2.stage_off = p->tiles[p->tile_of[t2.inst0]].stage_off;
+      p->tiles.insert(p->tiles.end(), extra.begin(), extra.end());
+      // largest tiles first (edges of the node range): in-order dispatch
+      // then fills the CUs longest-processing-time first
+      auto tcost = [&](const TileDesc& t) -> int64_t {
+        const tvbk_connectome& cn = d.conns[t.conn];
+        return (int64_t)(cn.row_ptr[t.node1] - cn.row_ptr[t.node0]) + (t.node1 - t.node0);
+      };
+      std::stable_sort(p->tiles.begin(), p->tiles.end(),
+                       [&](const TileDesc& x, const TileDesc& y) { return tcost(x) > tcost(y); });
+      for (size_t k = 0; k < p->tiles.size(); ++k)
+        if (p->tiles[k].node0 == 0) p->tile_of[p->tiles[k].inst0] = (int)k;
+    } else {
+      stage_len = 0;
+      for (TileDesc& td : p->tiles) { td.parts = 1; td.node0 = 0; td.node1 = p->conns[td.conn].n_node; }
+    }
   }
 
   // ---- derived parameters
@@ -632,7 +719,7 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   {
     size_t tot = p->conns.size() * sizeof(ConnDesc) + edges.size() * 4 + 16 + rp.size() * 4 +
                  (w64.size() * p->hsize + 16) + prm.size() * p->esize + (d.inst_stream ? (size_t)B * 8 : 0) +
-                 (size_t)ring_off * p->hsize + (size_t)img_off + (size_t)bacc_off * 8 + (size_t)B * 4 +
+                 (size_t)(ring_off + stage_len) * p->hsize + (size_t)img_off + (size_t)bacc_off * 8 + (size_t)B * 4 +
                  p->tiles.size() * sizeof(TileDesc) +
                  (p->uniform_rows ? (size_t)(p->conns[0].nnz + 4 * p->conns[0].n_node + TVBK_ROW_PAD) * (8 + p->hsize) +
                                         (size_t)(p->conns[0].n_node + 1) * 4 + 64 : 0);
@@ -741,6 +828,7 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
       if ((rc = dev_alloc(p, &p->d_bstock[m], (size_t)bst_len[m] * 8))) return bail(rc);
     }
   if ((rc = dev_alloc(p, &p->d_ring, (size_t)ring_off * p->hsize))) return bail(rc);
+  if ((rc = dev_alloc(p, &p->d_stage, (size_t)stage_len * p->hsize))) return bail(rc);
   if ((rc = dev_alloc(p, &p->d_img, (size_t)img_off))) return bail(rc);
   if ((rc = dev_alloc(p, &p->d_bacc, (size_t)bacc_off * 8))) return bail(rc);
   if ((rc = dev_alloc(p, &p->d_flags, (size_t)B * 4))) return bail(rc);
@@ -821,6 +909,13 @@ int tvbk_plan_shape(tvbk_plan* p, int32_t* kernel, int32_t* tile, int32_t* waves
   if (tile) *tile = p->T;
   if (waves) *waves = p->waves;
   if (lds_bytes) *lds_bytes = p->lds_total;
+  return TVBK_OK;
+}
+
+int tvbk_plan_tiles(tvbk_plan* p, int32_t* n_tiles, int32_t* split_k) {
+  if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
+  if (n_tiles) *n_tiles = (int32_t)p->tiles.size();
+  if (split_k) *split_k = p->split_k;
   return TVBK_OK;
 }
 
@@ -1019,6 +1114,7 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
   int64_t out_off[TVBK_MAX_MON] = {0};
   for (size_t k = 0; k < p->tiles.size(); ++k) {
     TileDesc& td = p->tiles[k];
+    if (td.node0 != 0) continue;            // further tiles of a split instance: copied below
     const int b0 = td.inst0;
     const int64_t cs = p->cur_step[b0];
     int64_t steps = n;
@@ -1058,6 +1154,14 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
     td.z_sk = (int64_t)p->mi.nsv * p->max_n * B;
     td.z_off = b0;
   }
+  for (TileDesc& td : p->tiles)
+    if (td.node0 != 0) {
+      const TileDesc& pt = p->tiles[p->tile_of[td.inst0]];
+      const int n0 = td.node0, n1 = td.node1;
+      td = pt;                             // the instance's call layout, this tile's nodes
+      td.node0 = n0;
+      td.node1 = n1;
+    }
   for (int m = 0; m < d.n_mon; ++m) {
     const int es = d.mon[m].kind == TVBK_MON_BOLD ? 8 : p->esize;
     int64_t cnt = 0;
@@ -1163,13 +1267,42 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
   a.dense_kb = p->dense_kb;
   a.dense_rb = p->dense_rb;
   a.lds_x = p->lds_x;
+  // image regions a split tile writes back (node-major, bytes per node)
+  a.n_img_reg = 0;
+  for (int m = 0; m < d.n_mon; ++m)
+    if (d.mon[m].kind == TVBK_MON_BOLD) {
+      a.img_reg_off[a.n_img_reg] = p->mon_lds_off[m];
+      a.img_reg_bpn[a.n_img_reg++] = d.mon[m].n_voi * T * 8;
+    }
+  a.img_reg_off[a.n_img_reg] = p->lds_state;
+  a.img_reg_bpn[a.n_img_reg++] = p->mi.nsv * T * p->esize;
+  for (int m = 0; m < d.n_mon; ++m)
+    if (d.mon[m].kind == TVBK_MON_TAVG) {
+      a.img_reg_off[a.n_img_reg] = p->mon_lds_off[m];
+      a.img_reg_bpn[a.n_img_reg++] = d.mon[m].n_voi * (p->esize == 8 ? 1 : 2) * T * p->esize;
+    }
+  a.stage = p->d_stage.p;
   HIPCHK(p, hipEventRecord(p->ev0, p->stream));
-  if (p->kernel == TVBK_KERNEL_DENSE_MFMA)
+  if (p->kernel == TVBK_KERNEL_DENSE_MFMA) {
+    a.k0 = 0;
+    a.k1 = (int32_t)n;
     HIPCHK(p, launch_dense(d.precision, d.model, a, (int)p->tiles.size(), p->nthreads, (size_t)p->lds_total,
                            p->stream));
-  else
-    HIPCHK(p, launch_step(d.precision, d.model, d.cfun, a, (int)p->tiles.size(), p->nthreads, (size_t)p->lds_total,
-                          p->stream));
+  } else {
+    // split plans run blocks of split_k steps, each followed by the commit
+    // of the split tiles' staged history; others run the call in one launch
+    const int64_t K = p->split_k > 0 ? p->split_k : std::max<int64_t>(n, 1);
+    for (int64_t k0 = 0; k0 < n; k0 += K) {
+      a.k0 = (int32_t)k0;
+      a.k1 = (int32_t)std::min<int64_t>(k0 + K, n);
+      HIPCHK(p, launch_step(d.precision, d.model, d.cfun, a, (int)p->tiles.size(), p->nthreads,
+                            (size_t)p->lds_total, p->stream));
+      if (p->split_k > 0)
+        HIPCHK(p, launch_commit(p->hsize, (const TileDesc*)p->d_tiles.p, (const ConnDesc*)p->d_conns.p,
+                                p->d_ring.p, p->d_stage.p, T, p->nch, (int)k0, (int)K, (int)p->tiles.size(),
+                                p->stream));
+    }
+  }
   // GlobalAverage / SpatialAverage of the call's node samples
   for (int m = 0; m < d.n_mon; ++m) {
     if (p->n_area[m] <= 0) continue;
@@ -1215,6 +1348,7 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
   p->have_timing = true;
   for (size_t k = 0; k < p->tiles.size(); ++k) {
     const TileDesc& td = p->tiles[k];
+    if (td.node0 != 0) continue;            // one tile per instance counts its steps
     for (int l = 0; l < td.n_inst; ++l) {
       const int b = td.inst0 + l;
       p->cur_step[b] += td.n_steps;

@@ -188,6 +188,42 @@ hipError_t launch_areas(int esize, int mode, const void* raw, void* out, const A
   return hipGetLastError();
 }
 
+// Commit of a block of a split plan: every split tile copies the history its
+// nodes produced at steps k0 .. k0 + nb - 1 of the call (staged as
+// [j][node][channel][T]) into ring slots (step0 + k0 + j) % H.
+template <class RH>
+__global__ void k_commit(const TileDesc* __restrict__ tiles, const ConnDesc* __restrict__ conns, RH* ring,
+                         const RH* __restrict__ stage, int T, int nch, int k0, int kn) {
+  const TileDesc td = tiles[blockIdx.y];
+  if (td.parts <= 1) return;
+  const int nb = (int)std::min<int64_t>(std::max<int64_t>(td.n_steps - k0, 0), kn);
+  const ConnDesc cd = conns[td.conn];
+  const int N = cd.n_node, H = cd.horizon;
+  const int rs = nch * T, nn = td.node1 - td.node0;
+  const int64_t per = (int64_t)nn * rs;
+  const int64_t total = per * nb;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(q / per);
+    const int64_t r = q - (int64_t)j * per;
+    const int i = td.node0 + (int)(r / rs), e = (int)(r % rs);
+    const int slot = (int)((td.step0 + k0 + j) % H);
+    ring[td.ring_off + ((int64_t)slot * N + i) * rs + e] = stage[td.stage_off + ((int64_t)j * N + i) * rs + e];
+  }
+}
+
+hipError_t launch_commit(int hsize, const TileDesc* tiles, const ConnDesc* conns, void* ring, const void* stage,
+                         int T, int nch, int k0, int kn, int ntiles, hipStream_t st) {
+  if (ntiles <= 0) return hipSuccess;
+  const dim3 grid(16, std::min(ntiles, 65535));
+  if (hsize == 8)
+    hipLaunchKernelGGL(k_commit<double>, grid, dim3(256), 0, st, tiles, conns, (double*)ring, (const double*)stage, T,
+                       nch, k0, kn);
+  else
+    hipLaunchKernelGGL(k_commit<float>, grid, dim3(256), 0, st, tiles, conns, (float*)ring, (const float*)stage, T,
+                       nch, k0, kn);
+  return hipGetLastError();
+}
+
 hipError_t launch_debug_slots(const uint32_t* edges, int nnz, int H, int64_t step, int32_t* out, hipStream_t st) {
   if (nnz <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_debug_slots, dim3((nnz + 255) / 256), dim3(256), 0, st, edges, nnz, H, step, out);

@@ -21,7 +21,12 @@ struct ConnDesc {
 
 // One workgroup's tile of instances and where its data lives.
 struct TileDesc {
-  int32_t conn, n_inst, inst0, pad;
+  int32_t conn, n_inst, inst0;
+  // target nodes [node0, node1) integrated by this tile; parts > 1: the
+  // instance is split over `parts` tiles, which stage their history writes
+  // at stage_off (elements) until the block's commit
+  int32_t node0, node1, parts;
+  int64_t stage_off;
   int64_t step0, n_steps;          // first step of this call, steps to run
   int64_t ring_off, img_off;       // ring elements, image bytes
   int64_t bacc_off[TVBK_KMAX_MON]; // Bold accumulators (doubles)
@@ -87,6 +92,13 @@ struct StepArgs {
   const double* stim_tp;
   uint32_t stim_mask;
   int32_t has_aff;
+  // this launch runs steps [k0, k1) of the call (a block of a split plan;
+  // 0 and the call's length otherwise); history staging of split tiles
+  int32_t k0, k1;
+  void* stage;
+  // image regions (byte offset, bytes per node) a split tile writes back
+  int32_t n_img_reg, pad_reg;
+  int32_t img_reg_off[1 + TVBK_KMAX_MON], img_reg_bpn[1 + TVBK_KMAX_MON];
   MonArgs mon[TVBK_KMAX_MON];
 };
 
@@ -121,6 +133,9 @@ hipError_t launch_dense(int precision, int model, const StepArgs& a, int ntiles,
 hipError_t launch_areas(int esize, int mode, const void* raw, void* out, const AreaJob* jobs, int n_jobs,
                         int max_count, int nvoi, int n_area, const int32_t* area_of, const double* weight,
                         hipStream_t st);
+// split tiles: staged history of a block's steps into the ring slots
+hipError_t launch_commit(int hsize, const TileDesc* tiles, const ConnDesc* conns, void* ring, const void* stage,
+                         int T, int nch, int k0, int kn, int ntiles, hipStream_t st);
 hipError_t launch_debug_slots(const uint32_t* edges, int nnz, int H, int64_t step, int32_t* out, hipStream_t st);
 // Balloon-Windkessel BOLD, one thread per series
 hipError_t launch_balloon(const void* x, int x_elem_bytes, int64_t x_stride, int64_t n_samples, int64_t n_series,

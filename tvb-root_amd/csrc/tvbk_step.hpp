@@ -182,7 +182,7 @@ __device__ __forceinline__ void monitors_node(const StepArgs& a, const KCfg& kc,
     if constexpr (EXT) {
       // afferent samples of step s - 1 are taken at step s (k >= 1) and by
       // the tail pass after the loop; the others at their own step
-      if (mo.src ? (k == 0) : aff_only) continue;
+      if (mo.src ? (k == a.k0) : aff_only) continue;
     }
     const StepInfo& si = kc.si[buf][m];
     const int nv = mo.nvoi;
@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ KCfg kc;
 
-  if (a.tiles[blockIdx.x].n_steps <= 0) return;   // uniform: whole block leaves
+  if (a.tiles[blockIdx.x].n_steps <= a.k0) return;   // uniform: whole block leaves
   // scalars of this tile and its connectome: loaded straight from global
   // memory (uniform addresses -> scalar loads, values stay in SGPRs)
   const TileDesc* tdg = a.tiles + blockIdx.x;
@@ -383,7 +383,12 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   const int T = a.T;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6, nwave = nthr >> 6;
-  const int nsteps = (int)td_n_steps;
+  const int nsteps = (int)td_n_steps;                            // steps of the whole call
+  // this launch: steps [kbeg, kfin) of the call, target nodes [node0, node1)
+  const int kbeg = a.k0, kfin = nsteps < a.k1 ? nsteps : a.k1;
+  const int node0 = tdg->node0, node1 = tdg->node1;
+  const bool staged = tdg->parts > 1;
+  const int64_t stage_off = tdg->stage_off;
   const int rstride = NCH * T;                                   // elements per (slot, node)
   const uint32_t cstride = (uint32_t)rstride * (uint32_t)sizeof(RH);  // bytes per node in a slot
   const uint32_t sstride = (uint32_t)N * cstride;                // bytes per slot (< 2^24)
@@ -398,7 +403,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
         mo.qe0 = p0 / mo.istep;
         mo.rb0 = (int)(p0 % mo.ii);
         mo.qb0 = p0 / mo.ii;
-        step_info(mo, 0, kc.si[0][m]);
+        step_info(mo, kbeg, kc.si[kbeg & 1][m]);
       }
       kc.mon[m] = mo;
     }
@@ -462,7 +467,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   RH* ring = reinterpret_cast<RH*>(a.ring) + td_ring_off;
 
   // ---- step counters (wave-uniform) --------------------------------------
-  int64_t s = td_step0;
+  int64_t s = td_step0 + kbeg;
   int smod = (int)((s - 1) % H);          // (s-1) % H ; s >= 1
   int bad = 0;
   const int P = N * NCH;
@@ -487,8 +492,9 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     return __umul24((uint32_t)sl, sstride);
   };
   auto phase_a = [&](auto getrec, const int32_t* RP, RH* cbuf) {
-    for (int q = wave * NG + g; q - g < P; q += nwave * NG) {
-      if (q < P) {
+    const int q0 = node0 * NCH, q1 = node1 * NCH;
+    for (int q = q0 + wave * NG + g; q - g < q1; q += nwave * NG) {
+      if (q < q1) {
         const int i = q / NCH, ch = q - (q / NCH) * NCH;
         const int e0 = RP[i];
         const int n = RP[i + 1] - e0;
@@ -629,7 +635,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
       const int q = j * 64 + lane;
       // lanes past the node's values read lane 0's (discarded)
       const uint32_t lane_off = q < QN ? (uint32_t)q * (uint32_t)sizeof(RH) : 0u;
-      for (int i = __builtin_amdgcn_readfirstlane(wave); i < N; i += nwave) {   // i uniform: scalar loads
+      for (int i = node0 + __builtin_amdgcn_readfirstlane(wave); i < node1; i += nwave) {   // i uniform: scalar loads
         RH v[RMAX];
         int e0;
         const int n = row_issue(RMc{}, i, base, lane_off, v, e0);
@@ -665,7 +671,8 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     for (int v = 0; v < NCP; ++v) cpv[v] = lane_param<R>(a, par, pst, NDP + v);
 #pragma unroll
     for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(i * NCT + v) * T + t], cpv);
-    RH* wr = ring + (wslot * N + i) * rstride + t;
+    RH* wr = staged ? reinterpret_cast<RH*>(a.stage) + stage_off + ((int64_t)(k - kbeg) * N + i) * rstride + t
+                    : ring + (wslot * N + i) * rstride + t;
     RH* st = cbuf + (i * NCT) * T + t;
     update_node<R, M, CF, RH, UM, PIPE == 0>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s_, buf, par,
                                              pst, c, bad,
@@ -733,8 +740,8 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   // with (simulator.py:311-316): the call's last sample needs the coupling of
   // the step after it, gathered by one more phase A (the next call gathers it
   // again) followed by those monitors only
-  const int kend = nsteps + (a.has_aff ? 1 : 0);
-  for (int k = 0; k < kend; ++k, ++s) {
+  const int kend = kfin + (a.has_aff ? 1 : 0);
+  for (int k = kbeg; k < kend; ++k, ++s) {
     const int buf = k & 1;
     RH* cbuf = s_c + (int64_t)buf * N * NCT * T;
     // ================= phase A: coupling of step s ========================
@@ -742,8 +749,8 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     else if (a.csr_in_lds) phase_a(rec_lds, l_rp, cbuf);
     else phase_a(rec_glb, g_rp, cbuf);
     __syncthreads();
-    if (k == nsteps) {
-      for (int i = tid / T; i < N; i += nthr / T) {
+    if (k == kfin) {
+      for (int i = node0 + tid / T; i < node1; i += nthr / T) {
         R c[NCT], x[NSV];
         R cpv[NCP > 0 ? NCP : 1];
 #pragma unroll
@@ -752,13 +759,13 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
         for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(i * NCT + v) * T + t], cpv);
 #pragma unroll
         for (int v = 0; v < NSV; ++v) x[v] = s_x[(i * NSV + v) * T + t];
-        monitors_node<R, M, CF, true>(a, kc, smem, i, t, T, N, live, nsteps, buf, x, c, true);
+        monitors_node<R, M, CF, true>(a, kc, smem, i, t, T, N, live, kfin, buf, x, c, true);
       }
       break;
     }
     // ================= phase B: integrate, write history, monitors =========
     const int wslot = (smod + 1 == H) ? 0 : smod + 1;   // s % H
-    for (int i = tid / T; i < N; i += nthr / T) b_item(i, cbuf, k, s, buf, wslot, false);
+    for (int i = node0 + tid / T; i < node1; i += nthr / T) b_item(i, cbuf, k, s, buf, wslot, false);
     // bookkeeping of step k+1 into the other buffer (read after the next
     // phase A barrier; its previous readers finished before this step's)
     if (tid == 0)
@@ -768,7 +775,18 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   }
   }
   __syncthreads();
-  copy_bytes(a.img + td_img_off, img, a.img_bytes);
+  if (node0 == 0 && node1 == N) {
+    copy_bytes(a.img + td_img_off, img, a.img_bytes);
+  } else {
+    // a split tile writes back its own nodes' part of every region
+    for (int r = 0; r < a.n_img_reg; ++r) {
+      const int64_t o = a.img_reg_off[r] + (int64_t)node0 * a.img_reg_bpn[r];
+      const int64_t nw = (int64_t)(node1 - node0) * a.img_reg_bpn[r] / 4;
+      uint32_t* d = reinterpret_cast<uint32_t*>(a.img + td_img_off + o);
+      const uint32_t* sr = reinterpret_cast<const uint32_t*>(img + o);
+      for (int64_t q = tid; q < nw; q += nthr) d[q] = sr[q];
+    }
+  }
   if (bad && live) atomicOr(a.flags + inst, 1);
 }
 
@@ -912,6 +930,7 @@ __global__ void k_init_uniform(InitArgs a) {
   using C = Cfun<M, CF>;
   constexpr int NSV = M::NSV, NCH = C::NCH, NDP = M::NDP;
   const TileDesc td = a.tiles[blockIdx.y];
+  if (td.node0 != 0) return;                 // one tile of a split instance fills it
   const ConnDesc cd = a.conns[td.conn];
   const int T = a.T, N = cd.n_node, H = cd.horizon;
   const int64_t per_tile = (int64_t)H * N * T;

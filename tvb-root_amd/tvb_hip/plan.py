@@ -329,6 +329,12 @@ class Plan:
             it = np.ascontiguousarray(np.asarray(interim, np.float64).reshape((ms.interim_istep, nv, N)))
         self._chk(self._lib.tvbk_import_monitor(self.handle, int(inst), int(m), L.ptr(st), L.ptr(it)))
 
+    def tiles(self):
+        """(workgroups per launch, block length of split instances or 0)."""
+        n, k = C.c_int32(), C.c_int32()
+        self._chk(self._lib.tvbk_plan_tiles(self.handle, C.byref(n), C.byref(k)))
+        return n.value, k.value
+
     def flags(self):
         f = np.zeros(self.n_inst, np.int32)
         self._chk(self._lib.tvbk_read_flags(self.handle, L.ptr(f)))

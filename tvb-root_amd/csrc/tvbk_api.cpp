@@ -626,7 +626,12 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
       cost[b] = cd.nnz + cd.n_node;
       total += cost[b];
     }
-    const int64_t target = std::max<int64_t>(total / (2 * (int64_t)std::max(n_cu, 1)), 4096);
+    // aim at `div` tiles of equal per-step cost per CU (TVBK_SPLIT_DIV,
+    // default 1: finer splits cost more in per-block launch work and idle
+    // lanes than they gain in balance)
+    const char* envd = std::getenv("TVBK_SPLIT_DIV");
+    const double div = std::max(0.05, envd ? std::atof(envd) : 1.0);
+    const int64_t target = std::max<int64_t>((int64_t)((double)total / (div * std::max(n_cu, 1))), 2048);
     int kmin = 1 << 30;
     std::vector<TileDesc> extra;
     for (int k = 0; k < ntiles && forced != 0; ++k) {

@@ -280,6 +280,65 @@ def run_c3(stream=None, n_inst=256, duration_scale=0.02):
     return info
 
 
+def c1_setup(seed=1920):
+    """Single stochastic instance: 192 nodes, 20 % density, lengths
+    U[5, 250] mm at 4 mm/ms, MontbrioPazoRoxin, Linear a = 0.5,
+    HeunStochastic dt 0.05 ms, nsig [0.01, 0.02], random initial history."""
+    from tvb_hip.connectome import idelays, synthetic
+    n, dt = 192, 0.05
+    w, L = synthetic(n, 0.20, 5.0, 250.0, seed=192)
+    d, H = idelays(L, 4.0, dt)
+    rng = np.random.default_rng(seed)
+    ic = np.empty((H, 2, n))
+    ic[:, 0] = rng.uniform(0.0, 2.0, size=(H, n))
+    ic[:, 1] = rng.uniform(-2.0, 1.5, size=(H, n))
+    return w, L, d, H, ic, rng
+
+
+def run_c1(n_steps=200000, check_steps=16000, precisions=("f32", "mixed", "f64")):
+    """The single-instance configuration: the full 10 s (200 000 steps) with
+    injected float64 normals resident on the device, TemporalAverage 1 ms +
+    Bold 720 ms, per precision; and the error against the fp64 oracle on
+    the first ``check_steps`` steps (one Bold emission)."""
+    import torch
+    from tvb_hip import _lib as L
+    from tvb_hip.plan import CSR, MonitorSpec, Plan
+    from oracle.tvb_oracle import OracleSim
+    w, Lm, d, H, ic, rng = c1_setup()
+    n = w.shape[0]
+    z = rng.standard_normal((n_steps, 2, n))
+    zd = torch.from_numpy(z.reshape(n_steps, 2, n, 1)).cuda()
+    p = np.array([[1.0, 0.0, 1.0, 15.0, -5.0, 0.0, 1.0, 0.0, 0.5, 0.0, 0.01, 0.02]])
+    o = OracleSim(w, Lm, 4.0, 0.05, "MontbrioPazoRoxin", coupling_params=dict(a=0.5), integrator="HeunStochastic",
+                  nsig=np.array([0.01, 0.02]), initial_conditions=ic,
+                  monitors=(("TemporalAverage", 1.0, None), ("Bold", 720.0, None)), mode="fp64")
+    (_, tref), (_, bref) = o.run(check_steps, noise=z[:check_steps])
+    out = {"nodes": n, "steps": n_steps, "check_steps": check_steps}
+    for name in precisions:
+        prec = {"f32": L.F32, "mixed": L.MIXED, "f64": L.F64}[name]
+        plan = Plan(model=L.MODEL_MPR, cfun=L.CFUN_LINEAR, scheme=L.HEUN_STO, precision=prec,
+                    noise=L.NOISE_INJECTED, conns=[CSR(w, d)], params=p, dt=0.05, obs_coef=np.eye(2),
+                    monitors=[MonitorSpec(L.MON_TAVG, 20, [0, 1], period=1.0), make_bold_spec(0.05, 720.0)],
+                    max_steps=n_steps, seed=0)
+        plan.set_history(0, ic, ic[-1], H - 1)
+        plan.set_noise(zd.data_ptr(), n_steps)
+        plan.sync()
+        t0 = time.perf_counter()
+        plan.run(n_steps)
+        plan.sync()
+        el = time.perf_counter() - t0
+        (_, ta), = plan.read(0)
+        (_, bo), = plan.read(1)
+        ntile, k = plan.tiles()
+        plan.close()
+        kc = check_steps // 20
+        out[name] = {"value": n * n_steps / el, "unit": "region-steps/s", "seconds": el, "tiles": ntile,
+                     "split_block_steps": k,
+                     "tavg_max_rel_vs_fp64_oracle": float(np.abs(ta[:kc] - tref[..., 0]).max() / np.abs(tref).max()),
+                     "bold_max_rel_vs_fp64_oracle": float(np.abs(bo[:1] - bref[..., 0]).max() / np.abs(bref).max())}
+    return out
+
+
 def run_c2_zero_delay(B=8192, n_steps=40000, chunk=10000, stream=None, warmup=1000, precision=None):
     """C2 with infinite conduction speed: every delay zero, horizon 1, so
     the coupling of a step is W . x(t) for all instances at once -- a
@@ -392,7 +451,7 @@ def main():
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=200000, help="oracle sample length (about 10-20 s of one host core)")
     ap.add_argument("--no-extras", action="store_true", help="skip CPU baseline, FP64 single path and error sample")
-    ap.add_argument("--workload", default="c2", choices=("c2", "c2z", "c3", "c4", "copy"))
+    ap.add_argument("--workload", default="c2", choices=("c2", "c1", "c2z", "c3", "c4", "copy"))
     ap.add_argument("--precision", default="f32", choices=("f32", "mixed"),
                     help="f32: float32 state, history and coupling, float64 Bold (the sweep's precision); "
                          "mixed: float32 history and coupling sums, float64 state (the reference's own mix)")
@@ -407,6 +466,10 @@ def main():
     from tvb_hip import _lib as L
     assert torch.cuda.is_available(), "no GPU"
     stream = torch.cuda.current_stream().cuda_stream
+    if args.workload == "c1":
+        print(json.dumps({"metric": "region-steps/s (single stochastic instance)", "workload": "c1",
+                          **run_c1()}), flush=True)
+        return
     if args.workload == "c3":
         print(json.dumps({"metric": "region-steps/s (uneven JansenRit batch)", "workload": "c3",
                           **run_c3(stream)}), flush=True)
@@ -507,6 +570,7 @@ def main():
         line["p50_latency_s"] = line["serving"]["p50_latency_s"]
         line["p99_latency_s"] = line["serving"]["p99_latency_s"]
         line["uneven_batch"] = run_c3(stream)
+        line["single_instance_c1"] = run_c1()
         line["zero_delay_mfma"] = run_c2_zero_delay(B, stream=stream, precision=prec)
         bal = run_balloon_on_c2(chunk, B)
         bal["share_of_chunk"] = bal["ms"] / avg_ms

@@ -237,7 +237,10 @@ __device__ inline R reduceat_regs_w(int n, const R* v, const W* w) {
 // pairwise blocks of eight are loaded as a batch before they are used.
 // ld(e) returns the raw value of entry e (zero past the row end), wt(e) its
 // weight; products are rounded before they are summed, as numpy does.
-template <class R, class LD, class WT>
+// PL: the next block's loads are issued before the current block is summed
+// (two blocks in flight, the sums in the same order); every entry read lies
+// below s + n + 8, as without it.
+template <class R, bool PL, class LD, class WT>
 __device__ inline R pw_leaf_blocked(int s, int n, LD& ld, WT& wt) {
   R x[8];
   if (n < 8) {
@@ -247,6 +250,31 @@ __device__ inline R pw_leaf_blocked(int s, int n, LD& ld, WT& wt) {
 #pragma unroll
     for (int k = 0; k < 7; ++k)
       if (k < n) res += wt(s + k) * x[k];
+    return res;
+  }
+  if constexpr (PL) {
+    R r[8], y[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = ld(s + k);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) y[k] = ld(s + 8 + k);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) r[k] = wt(s + k) * x[k];
+    const int nfull = n - (n & 7);
+    int i = 8;
+    for (; i < nfull; i += 8) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) x[k] = ld(s + i + 8 + k);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) r[k] += wt(s + i + k) * y[k];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) y[k] = x[k];
+    }
+    R res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    const int rem = n - nfull;
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+      if (k < rem) res += wt(s + i + k) * y[k];
     return res;
   }
   R r[8];
@@ -272,28 +300,28 @@ __device__ inline R pw_leaf_blocked(int s, int n, LD& ld, WT& wt) {
   return res;
 }
 
-template <int DEPTH, class R, class LD, class WT>
+template <int DEPTH, class R, bool PL, class LD, class WT>
 __device__ inline R pw_sum_blocked(int s, int n, LD& ld, WT& wt) {
   if constexpr (DEPTH == 0) {
-    return pw_leaf_blocked<R>(s, n, ld, wt);
+    return pw_leaf_blocked<R, PL>(s, n, ld, wt);
   } else {
-    if (n <= 128) return pw_leaf_blocked<R>(s, n, ld, wt);
+    if (n <= 128) return pw_leaf_blocked<R, PL>(s, n, ld, wt);
     int n2 = n / 2;
     n2 -= n2 & 7;
-    R a = pw_sum_blocked<DEPTH - 1, R>(s, n2, ld, wt);
-    R b = pw_sum_blocked<DEPTH - 1, R>(s + n2, n - n2, ld, wt);
+    R a = pw_sum_blocked<DEPTH - 1, R, PL>(s, n2, ld, wt);
+    R b = pw_sum_blocked<DEPTH - 1, R, PL>(s + n2, n - n2, ld, wt);
     return a + b;
   }
 }
 
-template <class R, class LD, class WT>
+template <class R, bool PL = false, class LD, class WT>
 __device__ inline R reduceat_row_blocked(int n, LD& ld, WT& wt) {
   if (n <= 0) return R(0);
   const R first = wt(0) * ld(0);
   if (n == 1) return first;
   auto ld1 = [&](int e) -> R { return ld(e + 1); };
   auto wt1 = [&](int e) -> R { return wt(e + 1); };
-  return first + pw_sum_blocked<3, R>(0, n - 1, ld1, wt1);
+  return first + pw_sum_blocked<3, R, PL>(0, n - 1, ld1, wt1);
 }
 
 // Raw buffer loads: 32-bit byte offsets from a per-tile base; offsets past

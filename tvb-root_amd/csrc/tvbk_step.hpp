@@ -500,7 +500,9 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
         const int n = RP[i + 1] - e0;
         const uint32_t lane_off = (uint32_t)(ch * T + t) * (uint32_t)sizeof(RH);
         RH acc;
-        if (n <= RMAX) {
+        // one path per wave (both give the same sums): a wave whose rows
+        // are not all short takes the blocked loads for every row
+        if (__all(n <= RMAX)) {
           // three passes: offsets, all loads back to back, then the weights
           // -- no load result is used before the row's last load issues
           // (float32 keeps the weights in registers; float64 re-reads them)
@@ -534,7 +536,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
             return buf_load<RH>(rsrc, e < n ? slot_off(r.delay) + r.coff + lane_off : 0xFFFFFFF0u);
           };
           auto wt = [&](int e) -> RH { return rec_weight<RH>(getrec(e0 + e)); };
-          acc = reduceat_row_blocked<RH>(n, ld, wt);
+          acc = reduceat_row_blocked<RH, true>(n, ld, wt);
         }
         // the raw sum; the coupling's post runs in the state precision in
         // phase B (coupling.py:166-167: post of the float32 sum in float64)

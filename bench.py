@@ -127,6 +127,81 @@ def cpu_baseline(w, L, G, sig, n_steps):
     return w.shape[0] * n_steps / el, el
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+_CPU_CHILD = r"""
+import sys, time
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from oracle.tvb_oracle import OracleSim
+w = np.load(sys.argv[2]); L = np.load(sys.argv[3]); n_steps = int(sys.argv[4]); seed = int(sys.argv[5])
+o = OracleSim(w, L, 5.0, 0.05, "MontbrioPazoRoxin", coupling_params=dict(a=1.0), integrator="HeunStochastic",
+              nsig=np.array([1e-3, 1e-3]), initial_conditions=None,
+              monitors=(("TemporalAverage", 1.0, None), ("Bold", 2000.0, None)), mode="faithful")
+z = np.random.default_rng(seed).standard_normal((n_steps, 2, w.shape[0]))
+t0 = time.perf_counter()
+o.run(n_steps, noise=z)
+print(time.perf_counter() - t0)
+"""
+
+
+def cpu_baseline_all_cores(w, L, n_steps, procs=None):
+    """The oracle on every host core the run may use: ``procs`` concurrent
+    single-threaded processes (at most 16), one independent instance of the
+    sweep each, as the framework runs one simulation per process; the rate
+    is all region-steps over the wall time of the slowest process."""
+    import subprocess
+    import tempfile
+    if procs is None:
+        procs = max(1, min(len(os.sched_getaffinity(0)), 16))
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    with tempfile.TemporaryDirectory() as tmp:
+        pw, pl = os.path.join(tmp, "w.npy"), os.path.join(tmp, "l.npy")
+        np.save(pw, w)
+        np.save(pl, L)
+        root = os.path.dirname(os.path.abspath(__file__))
+        t0 = time.perf_counter()
+        ps = [subprocess.Popen([sys.executable, "-c", _CPU_CHILD, root, pw, pl, str(n_steps), str(k)],
+                               stdout=subprocess.PIPE, stderr=subprocess.PIPE, env=env, text=True)
+              for k in range(procs)]
+        outs = [p.communicate() for p in ps]
+        wall = time.perf_counter() - t0
+    for p, (o, e) in zip(ps, outs):
+        if p.returncode != 0:
+            raise RuntimeError("oracle process failed: " + e[-400:])
+    loop = max(float(o.split()[-1]) for o, _ in outs)
+    return {"value": procs * w.shape[0] * n_steps / loop, "unit": "region-steps/s", "cores": procs, "kind": "port",
+            "sample": "%d processes x 1 instance x %d steps of the c2 workload, reference-faithful numpy oracle, "
+                      "1 thread each, slowest loop %.1f s (%.1f s wall with start-up), %s"
+                      % (procs, n_steps, loop, wall, cpu_model())}
+
+
+def gather_pattern_ceiling(roof):
+    """The device's rate for the step kernel's memory pattern (random
+    256-byte pieces, about 1/17 of the bytes written; tvbk_probe_gather)
+    and k_step's measured HBM rate (traffic per launch / launch time) as a
+    share of it; the same pattern with 1 KiB pieces for comparison."""
+    from tvb_hip import _lib as L
+    r = {"unit": "GB/s",
+         "reads_256B": L.probe_gather(256, False) / 1e9,
+         "reads_256B_writes_1_17": L.probe_gather(256, True) / 1e9,
+         "reads_1KiB_writes_1_17": L.probe_gather(1024, True) / 1e9}
+    if roof.get("traffic"):
+        rate = roof["traffic"] / (roof["avg_launch_ms"] * 1e-3) / 1e9
+        r["k_step_hbm_rate"] = rate
+        r["k_step_share"] = rate / r["reads_256B_writes_1_17"]
+    return r
+
+
 def c2_rate(w, d, G, sig, precision, n_steps=40000, chunk=20000, stream=None):
     """Throughput of the full sweep in another precision: launches of
     ``chunk`` steps after a warm-up, HIP-event launch times."""
@@ -549,6 +624,7 @@ def main():
     }
     plan.close()
     if not args.no_extras:
+        line["gather_pattern_ceiling"] = gather_pattern_ceiling(line["roofline"])
         v64, el64, shape64 = fp64_single_instance(w, d, 1.0, 1e-3, 40000)
         line["fp64_single_instance"] = {"value": v64, "unit": "region-steps/s", "steps": 40000,
                                         "kernel": list(shape64)}
@@ -578,8 +654,9 @@ def main():
         cpu_rate, cpu_el = cpu_baseline(w, Lmat, 1.0, 1e-3, args.cpu_steps)
         line["cpu_baseline"] = {"value": cpu_rate, "unit": "region-steps/s", "cores": 1, "kind": "port",
                                 "sample": "1 instance x %d steps (%.1f s simulated) of the c2 workload, "
-                                          "reference-faithful numpy oracle, 1 thread, %.1f s wall"
-                                          % (args.cpu_steps, args.cpu_steps * dt / 1000.0, cpu_el)}
+                                          "reference-faithful numpy oracle, 1 thread, %.1f s wall, %s"
+                                          % (args.cpu_steps, args.cpu_steps * dt / 1000.0, cpu_el, cpu_model())}
+        line["cpu_baseline_all_cores"] = cpu_baseline_all_cores(w, Lmat, args.cpu_steps // 4)
     print(json.dumps(line), flush=True)
 
 

@@ -350,6 +350,18 @@ int tvbk_balloon(const void *x, int32_t x_elem_bytes, int64_t x_stride,
                  const double *params, int32_t flags, int32_t start,
                  double *state, double *y, void *stream);
 
+/* Calibration of the delayed gather's HBM pattern; no counterpart in the
+ * reference (it measures the device, for the step kernel's roofline).
+ * Waves read 256-byte pieces in segments of piece_bytes (256 .. 4096) at
+ * random segment-aligned offsets of a scratch buffer of `bytes` (64 MiB ..
+ * 64 GiB, allocated and freed here), 16 pieces in flight per wave, one
+ * 16-wave workgroup per CU; with write = 1 one segment is stored per 16
+ * segments read (1/17 of the bytes, about the share of the ring writes).
+ * *bytes_per_s receives the read + write rate of the best of three
+ * launches of about 2 GiB.  Synchronous (default stream). */
+int tvbk_probe_gather(int64_t bytes, int32_t piece_bytes, int32_t write,
+                      double *bytes_per_s);
+
 const char *tvbk_last_error(tvbk_plan *plan);
 
 #ifdef __cplusplus

@@ -57,6 +57,15 @@ def test_plan_fails_loudly_without_device():
         HipBackend().build_plan(sim)
 
 
+def test_probe_gather_rejects_bad_arguments():
+    """Argument checks of the calibration entry point run before any device
+    call."""
+    from tvb_hip import _lib as L
+    for piece, write, nbytes in ((300, False, 1 << 30), (256, False, 1 << 20), (8192, True, 1 << 30)):
+        with pytest.raises(L.TvbkError):
+            L.probe_gather(piece, write, nbytes)
+
+
 def test_missing_library_raises(monkeypatch, tmp_path):
     from tvb_hip import _lib
     monkeypatch.setattr(_lib, "_lib", None)

@@ -1403,6 +1403,53 @@ int tvbk_balloon(const void* x, int32_t x_elem_bytes, int64_t x_stride, int64_t 
   return TVBK_OK;
 }
 
+int tvbk_probe_gather(int64_t bytes, int32_t piece_bytes, int32_t write, double* bytes_per_s) {
+  if (!bytes_per_s) return fail(nullptr, TVBK_EINVAL, "null argument");
+  if (bytes < ((int64_t)64 << 20) || bytes > ((int64_t)64 << 30))
+    return fail(nullptr, TVBK_EINVAL, "probe buffer must hold 64 MiB to 64 GiB");
+  const int k = piece_bytes / 256;
+  if (piece_bytes % 256 || (k != 1 && k != 2 && k != 4 && k != 8 && k != 16))
+    return fail(nullptr, TVBK_EINVAL, "piece_bytes must be 256, 512, 1024, 2048 or 4096");
+  if (write != 0 && write != 1) return fail(nullptr, TVBK_EINVAL, "write must be 0 or 1");
+  int dev = 0, n_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return fail(nullptr, TVBK_EHIP, std::string("device: ") + hipGetErrorString(e));
+  const int grid = n_cu > 0 ? n_cu : 1;          // one 16-wave workgroup per CU, as the step kernel
+  // about 2 GiB read per launch; a multiple of 16 batches (every k divides it)
+  const double per_batch = (double)grid * 16 * 16 * 256;
+  int iters = (int)((double)(1u << 31) / per_batch);
+  iters = iters < 16 ? 16 : iters - iters % 16;
+  float *buf = nullptr, *out = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float best = 0.f;
+  e = hipMalloc(&buf, (size_t)bytes);
+  if (e == hipSuccess) e = hipMalloc(&out, (size_t)grid * 1024 * sizeof(float));
+  if (e == hipSuccess) e = hipMemset(buf, 0, (size_t)bytes);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = launch_probe_gather(buf, (size_t)bytes / 4, k, write != 0, grid, 16, out, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  for (int r = 0; r < 3 && e == hipSuccess; ++r) {
+    float ms = 0.f;
+    e = hipEventRecord(e0, nullptr);
+    if (e == hipSuccess) e = launch_probe_gather(buf, (size_t)bytes / 4, k, write != 0, grid, iters, out, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess && (r == 0 || ms < best)) best = ms;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (buf) (void)hipFree(buf);
+  if (out) (void)hipFree(out);
+  if (e != hipSuccess) return fail(nullptr, TVBK_EHIP, std::string("probe: ") + hipGetErrorString(e));
+  const double rd = (double)grid * 16 * iters * 16 * 256;
+  const double wr = write ? (double)grid * 16 * (iters / k) * k * 256 : 0.0;
+  *bytes_per_s = (rd + wr) / (best * 1e-3);
+  return TVBK_OK;
+}
+
 int tvbk_monitor_layout(tvbk_plan* p, int32_t mon, int32_t* layout, int64_t* total_bytes, int64_t* offset,
                         int64_t* count, int64_t* first_step) {
   if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");

@@ -141,6 +141,11 @@ hipError_t launch_debug_slots(const uint32_t* edges, int nnz, int H, int64_t ste
 hipError_t launch_balloon(const void* x, int x_elem_bytes, int64_t x_stride, int64_t n_samples, int64_t n_series,
                           const double* par, int linear, int classical, int start, int normalize,
                           double sample_period_ms, double* state, double* y, hipStream_t st);
+// the gather-pattern calibration: 16 reads of 256-byte pieces per wave and
+// batch in segments of k pieces at random offsets of buf (n_float floats),
+// with wr one segment stored every k batches; out holds grid * 1024 floats
+hipError_t launch_probe_gather(float* buf, size_t n_float, int k, bool wr, int grid, int iters, float* out,
+                               hipStream_t st);
 // scatter a compact history [H][nch][N] (history precision) / state
 // [nsv][N] (state precision) into lane t of a tile, and the reverse gather.
 hipError_t launch_scatter(int precision, bool gather, void* ring, unsigned char* img_state, void* hist,

@@ -98,6 +98,7 @@ def _bind(lib):
         "tvbk_last_run_ms": ([P, P], C.c_int),
         "tvbk_balloon": ([P, C.c_int32, C.c_int64, C.c_int64, C.c_int64, C.c_double, P, C.c_int32, C.c_int32,
                           P, P, P], C.c_int),
+        "tvbk_probe_gather": ([C.c_int64, C.c_int32, C.c_int32, P], C.c_int),
         "tvbk_last_error": ([P], C.c_char_p),
     }
     for name, (argtypes, restype) in sig.items():
@@ -114,7 +115,8 @@ EXPORTED = (
     "tvbk_set_steps", "tvbk_set_stimulus", "tvbk_sync", "tvbk_monitor_layout", "tvbk_read_monitor",
     "tvbk_monitor_device_ptr", "tvbk_export_state", "tvbk_export_monitor", "tvbk_import_monitor",
     "tvbk_read_flags",
-    "tvbk_debug_slots", "tvbk_plan_shape", "tvbk_plan_tiles", "tvbk_last_run_ms", "tvbk_balloon", "tvbk_last_error")
+    "tvbk_debug_slots", "tvbk_plan_shape", "tvbk_plan_tiles", "tvbk_last_run_ms", "tvbk_balloon",
+    "tvbk_probe_gather", "tvbk_last_error")
 
 
 def load():
@@ -156,6 +158,17 @@ def ptr(a):
         return None
     assert a.flags["C_CONTIGUOUS"]
     return a.ctypes.data
+
+
+def probe_gather(piece_bytes=256, write=False, nbytes=3200 << 20):
+    """Read (+ write) bytes/s of the gather's access pattern on this device
+    (tvbk_probe_gather): 256-byte pieces read in segments of ``piece_bytes``
+    at random offsets of an ``nbytes`` buffer, with ``write`` one segment
+    stored per 16 read."""
+    lib = load()
+    r = C.c_double(0.0)
+    check(lib.tvbk_probe_gather(int(nbytes), int(piece_bytes), 1 if write else 0, C.byref(r)))
+    return r.value
 
 
 def stream_create():

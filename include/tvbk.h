@@ -314,13 +314,19 @@ int tvbk_read_flags(tvbk_plan *plan, int32_t *flags);
  * ((step - 1 - d + H) % H, history.py:252) into slots[nnz] (int32). */
 int tvbk_debug_slots(tvbk_plan *plan, int32_t conn, int64_t step, int32_t *slots);
 
-/* Kernel family, tile and waves chosen for the plan; LDS bytes per block. */
 /* Number of workgroups (tiles) of a launch and the block length of a plan
  * whose large edge-lane instances are split over several tiles by node
  * range (0: not split).  No reference counterpart (a property of this
  * backend's launch). */
 int tvbk_plan_tiles(tvbk_plan *plan, int32_t *n_tiles, int32_t *split_k);
 
+/* Steps per block of the time-blocked sequential loop (the coupling of up
+ * to d_min steps gathered at once, then each node integrated through them;
+ * 0: one step at a time).  No reference counterpart (a property of this
+ * backend's launch). */
+int tvbk_plan_time_block(tvbk_plan *plan, int32_t *steps);
+
+/* Kernel family, tile and waves chosen for the plan; LDS bytes per block. */
 int tvbk_plan_shape(tvbk_plan *plan, int32_t *kernel, int32_t *tile,
                     int32_t *waves, int32_t *lds_bytes);
 

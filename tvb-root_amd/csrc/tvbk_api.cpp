@@ -129,6 +129,9 @@ struct tvbk_plan {
   std::vector<ConnDesc> conns;
   // LDS layout
   int lds_bold = 0, lds_state = 0, lds_c = 0, lds_edge = 0, lds_w = 0, lds_rp = 0, lds_par = 0, lds_total = 0;
+  // time-blocked sequential loop: steps per block (0: one step at a time)
+  // and the LDS offset of the block's monitor step records
+  int tb_k = 0, lds_si = 0;
   int64_t img_bytes = 0;
   bool csr_in_lds = false;
   bool uniform_rows = false;   // batch-lane, n_channel * T a multiple of 32
@@ -290,8 +293,10 @@ static int layout_lds(tvbk_plan* p, int T, int64_t max_lds, bool allow_csr) {
   p->lds_par = (int)off;                       // per-lane parameters [prow][T]
   off = align_up(off + (int64_t)p->prow * T * p->esize, 16);
   p->lds_c = (int)off;
-  off += 2 * (int64_t)N * p->nct * T * p->hsize;
+  off += (int64_t)std::max(2, p->tb_k) * N * p->nct * T * p->hsize;
   off = align_up(off, 16);
+  p->lds_si = (int)off;
+  if (p->tb_k > 1) off = align_up(off + (int64_t)p->tb_k * TVBK_KMAX_MON * TVBK_STEPINFO_BYTES, 16);
   // CSR staged as 16-byte records {col offset, delay, weight bits}
   const int64_t ne = p->max_nnz + TVBK_ROW_PAD;
   const int64_t csr = ne * 16 + align_up((N + 1) * 4, 16);
@@ -529,6 +534,29 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   // scalar cache (no CSR in LDS)
   p->uniform_rows = kernel == TVBK_KERNEL_BATCH_LANE && (p->nch * T) % 32 == 0;
   if (p->uniform_rows) layout_lds(p, T, max_lds, false);
+  // time-blocked sequential loop: plans that cannot take the pipelined loop,
+  // with every delay >= 2 and no afferent monitor, gather the coupling of
+  // up to TVBK_TB_MAX (<= d_min) steps at once -- as many coupling buffers
+  // as LDS holds without moving the CSR out of it.  TVBK_TIME_BLOCK: unset =
+  // automatic, 0 or 1 = off, n = blocks of at most n steps
+  p->tb_k = 0;
+  int tb_cap = TVBK_TB_MAX;
+  if (const char* env = std::getenv("TVBK_TIME_BLOCK"))
+    if (*env) tb_cap = std::max(0, std::min(TVBK_TB_MAX, std::atoi(env)));
+  if (kernel != TVBK_KERNEL_DENSE_MFMA && !(p->uniform_rows && p->nch * T <= 64) && !any_aff && tb_cap >= 2) {
+    int dmin = tb_cap;
+    for (const ConnDesc& cd : p->conns)
+      if (cd.nnz > 0) dmin = std::min(dmin, cd.d_min);
+    if (dmin >= 2) {
+      const bool csr0 = p->csr_in_lds, allow = !p->uniform_rows;
+      for (int K = dmin; K >= 2; --K) {
+        p->tb_k = K;
+        if (layout_lds(p, T, max_lds, allow) == TVBK_OK && p->csr_in_lds == csr0) break;
+        p->tb_k = 0;
+      }
+      if (!p->tb_k) layout_lds(p, T, max_lds, allow);
+    }
+  }
   {
     const int P = p->max_n * std::max(p->nch, 1);
     int nw;
@@ -917,6 +945,12 @@ int tvbk_plan_shape(tvbk_plan* p, int32_t* kernel, int32_t* tile, int32_t* waves
   return TVBK_OK;
 }
 
+int tvbk_plan_time_block(tvbk_plan* p, int32_t* steps) {
+  if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
+  if (steps) *steps = p->tb_k;
+  return TVBK_OK;
+}
+
 int tvbk_plan_tiles(tvbk_plan* p, int32_t* n_tiles, int32_t* split_k) {
   if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
   if (n_tiles) *n_tiles = (int32_t)p->tiles.size();
@@ -1234,6 +1268,8 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
   a.lds_bold = p->lds_bold;
   a.lds_state = p->lds_state;
   a.lds_c = p->lds_c;
+  a.lds_si = p->lds_si;
+  a.tb_k = p->tb_k;
   a.lds_par = p->lds_par;
   a.par_umask = p->par_umask;
   for (int k = 0; k < TVBK_KMAX_PAR; ++k) a.par_uval[k] = p->par_uval[k];

@@ -18,6 +18,10 @@ struct ConnDesc {
 // CSR arrays (global and LDS copies) carry this many padding entries past
 // the last edge so that rows can be read in fixed-width blocks.
 #define TVBK_ROW_PAD 32
+// longest block of the time-blocked sequential loop, and the bytes of one
+// monitor step record (StepInfo) it keeps per step and monitor in LDS
+#define TVBK_TB_MAX 32
+#define TVBK_STEPINFO_BYTES 48
 
 // One workgroup's tile of instances and where its data lives.
 struct TileDesc {
@@ -100,6 +104,9 @@ struct StepArgs {
   int32_t n_img_reg, pad_reg;
   int32_t img_reg_off[1 + TVBK_KMAX_MON], img_reg_bpn[1 + TVBK_KMAX_MON];
   MonArgs mon[TVBK_KMAX_MON];
+  // time-blocked sequential loop: steps per block (<= 1: off) and the LDS
+  // offset of its monitor step records [tb_k][TVBK_KMAX_MON]
+  int32_t tb_k, lds_si;
 };
 
 // Node -> area reduction of one monitor's subsampled output (GlobalAverage,

@@ -56,6 +56,7 @@ struct StepInfo {
   // fir_age0 + j * rat
   int32_t fir_n, fir_slot0, fir_age0, pad2;
 };
+static_assert(sizeof(StepInfo) == TVBK_STEPINFO_BYTES, "StepInfo size is part of the LDS layout");
 
 struct KCfg {
   TileDesc td;
@@ -150,8 +151,8 @@ __device__ inline void stage_params(const StepArgs& a, unsigned char* smem, int 
 // present; aff_only: the pass after the last step, afferent monitors only.
 template <class R, class M, int CF, bool EXT>
 __device__ __forceinline__ void monitors_node(const StepArgs& a, const KCfg& kc, unsigned char* smem, int i, int t,
-                                              int T, int N, bool live, int k, int buf, const R* x, const R* c,
-                                              bool aff_only) {
+                                              int T, int N, bool live, int k, const StepInfo* sir, const R* x,
+                                              const R* c, bool aff_only) {
   using C = Cfun<M, CF>;
   constexpr int NSV = M::NSV, NCT = C::NCT;
   // this tile's descriptor: read-only during the kernel, scalar loads
@@ -184,7 +185,7 @@ __device__ __forceinline__ void monitors_node(const StepArgs& a, const KCfg& kc,
       // the tail pass after the loop; the others at their own step
       if (mo.src ? (k == a.k0) : aff_only) continue;
     }
-    const StepInfo& si = kc.si[buf][m];
+    const StepInfo& si = sir[m];
     const int nv = mo.nvoi;
     const bool emit = si.emit != 0;
     if (mo.kind == MON_RAW || mo.kind == MON_TAVG) {
@@ -283,7 +284,7 @@ __device__ __forceinline__ void monitors_node(const StepArgs& a, const KCfg& kc,
 template <class R, class M, int CF, class NR, bool UM = false, bool EXT = false, class SINK>
 __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, unsigned char* smem, R* s_x, int i,
                                             int t, int T, int N, bool live, int inst, uint64_t stream, int k,
-                                            int64_t s, int buf, const R* par, int pst, const R* c,
+                                            int64_t s, const StepInfo* sir, const R* par, int pst, const R* c,
                                             int& bad, SINK sink) {
   using C = Cfun<M, CF>;
   constexpr int NSV = M::NSV, NDP = M::NDP, NCP = C::NCP, NCH = C::NCH;
@@ -349,7 +350,7 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) sink(ch, C::template channel<R>(x, ch, cp));
 
-  monitors_node<R, M, CF, EXT>(a, kc, smem, i, t, T, N, live, k, buf, x, c, false);
+  monitors_node<R, M, CF, EXT>(a, kc, smem, i, t, T, N, live, k, sir, x, c, false);
 }
 
 // RH: history / weight / coupling-sum type, R: state, integration and
@@ -358,8 +359,9 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
 // and twice the registers.
 // PIPE = 1: the pipelined loop only (wave-uniform rows, every delay >= 1,
 // n_channel * T <= 64); PIPE = 0: the sequential loop with every gather
-// variant.  Separate instantiations keep each loop's register allocation to
-// itself.  UM: every model parameter is a plan constant (a coupling or noise
+// variant; PIPE = 2: the time-blocked sequential loop (blocks of up to
+// d_min steps, no afferent monitor).  Separate instantiations keep each
+// loop's register allocation to itself.  UM: every model parameter is a plan constant (a coupling or noise
 // sweep) and is read from the kernel arguments.
 template <class RH, class R, class M, int CF, int PIPE, bool UM>
 __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs a) {
@@ -486,15 +488,24 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   using RPc = std::integral_constant<int, RP>;
   const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(ring, (uint32_t)H * sstride);
   const int rmax = cd_max_row < RMAX ? cd_max_row : RMAX;
-  auto slot_off = [&](uint32_t delay) -> uint32_t {
-    int sl = smod - (int)delay;
+  // byte offset of the slot `delay` steps before the step whose (s - 1) % H
+  // is sm
+  auto slot_off = [&](int sm, uint32_t delay) -> uint32_t {
+    int sl = sm - (int)delay;
     sl += (sl < 0) ? H : 0;
     return __umul24((uint32_t)sl, sstride);
   };
-  auto phase_a = [&](auto getrec, const int32_t* RP, RH* cbuf) {
-    const int q0 = node0 * NCH, q1 = node1 * NCH;
-    for (int q = q0 + wave * NG + g; q - g < q1; q += nwave * NG) {
-      if (q < q1) {
+  // phase A of nk consecutive steps, the first with (s - 1) % H = smod
+  // (nk <= d_min: every slot read was written before the first of them):
+  // items (step j, node-channel pair q), step j's sums into cbuf0 + j * cstep
+  const int64_t cstep = (int64_t)N * NCT * T;
+  auto phase_a = [&](auto getrec, const int32_t* RP, RH* cbuf0, int nk) {
+    const int q0 = node0 * NCH, P = (node1 - node0) * NCH, tot = nk * P;
+    for (int it = wave * NG + g; it - g < tot; it += nwave * NG) {
+      if (it < tot) {
+        const int j = it / P, q = q0 + (it - j * P);
+        const int sm = smod + j < H ? smod + j : smod + j - H;
+        RH* cbuf = cbuf0 + j * cstep;
         const int i = q / NCH, ch = q - (q / NCH) * NCH;
         const int e0 = RP[i];
         const int n = RP[i + 1] - e0;
@@ -513,7 +524,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
           for (int e = 0; e < RMAX; ++e) {
             if (e < rmax) {
               const EdgeRec r = getrec(e0 + e);
-              off[e] = (e < n) ? slot_off(r.delay) + r.coff + lane_off : 0xFFFFFFF0u;
+              off[e] = (e < n) ? slot_off(sm, r.delay) + r.coff + lane_off : 0xFFFFFFF0u;
               if constexpr (KEEPW) wv[e] = rec_weight<RH>(r);
             } else {
               off[e] = 0xFFFFFFF0u;
@@ -533,7 +544,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
           // long rows: the same order in blocks of eight loads
           auto ld = [&](int e) -> RH {
             const EdgeRec r = getrec(e0 + e);
-            return buf_load<RH>(rsrc, e < n ? slot_off(r.delay) + r.coff + lane_off : 0xFFFFFFF0u);
+            return buf_load<RH>(rsrc, e < n ? slot_off(sm, r.delay) + r.coff + lane_off : 0xFFFFFFF0u);
           };
           auto wt = [&](int e) -> RH { return rec_weight<RH>(getrec(e0 + e)); };
           acc = reduceat_row_blocked<RH, true>(n, ld, wt);
@@ -631,17 +642,21 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
       cbuf[(i * NCT + (CF == CF_SIGMOID_JR ? 0 : ch)) * T + tq] = acc;
     }
   };
-  auto phase_a_uni = [&](RH* cbuf) {
-    const uint32_t base = __builtin_amdgcn_readfirstlane((uint32_t)smod * sstride);   // slot (s - 1) % H
+  // the same for wave-uniform rows: items (step kk, node i), one per wave
+  auto phase_a_uni = [&](RH* cbuf0, int nk) {
+    const int nn = node1 - node0;
     for (int j = 0; j * 64 < QN; ++j) {                 // uniform trip count
       const int q = j * 64 + lane;
       // lanes past the node's values read lane 0's (discarded)
       const uint32_t lane_off = q < QN ? (uint32_t)q * (uint32_t)sizeof(RH) : 0u;
-      for (int i = node0 + __builtin_amdgcn_readfirstlane(wave); i < node1; i += nwave) {   // i uniform: scalar loads
+      for (int it = __builtin_amdgcn_readfirstlane(wave); it < nk * nn; it += nwave) {   // uniform: scalar loads
+        const int kk = it / nn, i = node0 + (it - kk * nn);
+        const int sm = smod + kk < H ? smod + kk : smod + kk - H;
+        const uint32_t base = __builtin_amdgcn_readfirstlane((uint32_t)sm * sstride);   // slot (s - 1 + kk) % H
         RH v[RMAX];
         int e0;
         const int n = row_issue(RMc{}, i, base, lane_off, v, e0);
-        row_store(cbuf, i, q, row_finish(RMc{}, e0, n, base, lane_off, v));
+        row_store(cbuf0 + kk * cstep, i, q, row_finish(RMc{}, e0, n, base, lane_off, v));
       }
     }
   };
@@ -666,7 +681,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   // With `stage`, the new history channels go to this item's entries of the
   // coupling buffer it has just read (NCT == NCH) instead of the ring, and
   // ring_out() stores them later.
-  auto b_item = [&](int i, RH* cbuf, int k, int64_t s_, int buf, int wslot, bool stage) {
+  auto b_item = [&](int i, RH* cbuf, int k, int64_t s_, const StepInfo* sir, int wslot, bool stage) {
     R c[NCT];
     R cpv[NCP > 0 ? NCP : 1];
 #pragma unroll
@@ -676,7 +691,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     RH* wr = staged ? reinterpret_cast<RH*>(a.stage) + stage_off + ((int64_t)(k - kbeg) * N + i) * rstride + t
                     : ring + (wslot * N + i) * rstride + t;
     RH* st = cbuf + (i * NCT) * T + t;
-    update_node<R, M, CF, RH, UM, PIPE == 0>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s_, buf, par,
+    update_node<R, M, CF, RH, UM, PIPE != 1>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s_, sir, par,
                                              pst, c, bad,
                                   [&](int ch, R v) {
                                     if (stage) st[ch * T] = (RH)v;
@@ -698,7 +713,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   // its updates of phase B(s) -- a row's loads are in flight while the wave
   // integrates -- and one barrier per step orders both against the next.
   if constexpr (PIPE == 1) {
-    phase_a_uni(s_c);                                   // coupling of the first step
+    phase_a_uni(s_c, 1);                                // coupling of the first step
     __syncthreads();
     // rows from the last wave down: the first waves carry the extra phase B
     // pass when N * T is not a multiple of the block
@@ -724,7 +739,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
       auto body = [&](int u, RH* vf, int ef, int nf, RH* vi, int& ei, int& ni) {
         if (u + 1 < nrow) ni = row_issue(RPc{}, w0 + (u + 1) * nwave, base_n, lane_off, vi, ei);
         const int ib = tid / T + u * (nthr / T);
-        if (ib < N) b_item(ib, cbuf, k, s, buf, wslot, NCT == NCH);
+        if (ib < N) b_item(ib, cbuf, k, s, kc.si[buf], wslot, NCT == NCH);
         if (u < nrow) row_store(cnext, w0 + u * nwave, lane, row_finish(RPc{}, ef, nf, base_n, lane_off, vf));
       };
       for (int u = 0; u < nu; u += 2) {
@@ -737,6 +752,34 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
       smod = wslot;
       __syncthreads();
     }
+  } else if constexpr (PIPE == 2) {
+    // ---- time-blocked sequential loop: a block of nk <= d_min steps reads
+    // only history written before the block, so phase A gathers the
+    // coupling of all nk steps at once; then each thread integrates its
+    // nodes through the nk steps (a node's update needs only its own state
+    // and coupling).  Two barriers per block; the monitors' step records of
+    // the block are kept in LDS.
+    StepInfo* si_blk = reinterpret_cast<StepInfo*>(smem + a.lds_si);
+    for (int kb = kbeg; kb < kfin; kb += a.tb_k) {
+      const int nk = kfin - kb < a.tb_k ? kfin - kb : a.tb_k;
+      if (tid == 0)
+        for (int j = 0; j < nk; ++j)
+          for (int m = 0; m < a.n_mon; ++m) step_info(kc.mon[m], kb + j, si_blk[j * TVBK_KMAX_MON + m]);
+      if (a.uniform_rows) phase_a_uni(s_c, nk);
+      else if (a.csr_in_lds) phase_a(rec_lds, l_rp, s_c, nk);
+      else phase_a(rec_glb, g_rp, s_c, nk);
+      __syncthreads();
+      for (int i = node0 + tid / T; i < node1; i += nthr / T) {
+        int ws = smod;
+        for (int j = 0; j < nk; ++j) {
+          ws = (ws + 1 == H) ? 0 : ws + 1;                // (s + j) % H
+          b_item(i, s_c + j * cstep, kb + j, s + j, si_blk + j * TVBK_KMAX_MON, ws, false);
+        }
+      }
+      s += nk;
+      smod = smod + nk < H ? smod + nk : smod + nk - H;
+      __syncthreads();
+    }
   } else {
   // afferent monitors sample, at step s, the coupling step s + 1 integrates
   // with (simulator.py:311-316): the call's last sample needs the coupling of
@@ -747,9 +790,9 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     const int buf = k & 1;
     RH* cbuf = s_c + (int64_t)buf * N * NCT * T;
     // ================= phase A: coupling of step s ========================
-    if (a.uniform_rows) phase_a_uni(cbuf);
-    else if (a.csr_in_lds) phase_a(rec_lds, l_rp, cbuf);
-    else phase_a(rec_glb, g_rp, cbuf);
+    if (a.uniform_rows) phase_a_uni(cbuf, 1);
+    else if (a.csr_in_lds) phase_a(rec_lds, l_rp, cbuf, 1);
+    else phase_a(rec_glb, g_rp, cbuf, 1);
     __syncthreads();
     if (k == kfin) {
       for (int i = node0 + tid / T; i < node1; i += nthr / T) {
@@ -761,13 +804,13 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
         for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(i * NCT + v) * T + t], cpv);
 #pragma unroll
         for (int v = 0; v < NSV; ++v) x[v] = s_x[(i * NSV + v) * T + t];
-        monitors_node<R, M, CF, true>(a, kc, smem, i, t, T, N, live, kfin, buf, x, c, true);
+        monitors_node<R, M, CF, true>(a, kc, smem, i, t, T, N, live, kfin, kc.si[buf], x, c, true);
       }
       break;
     }
     // ================= phase B: integrate, write history, monitors =========
     const int wslot = (smod + 1 == H) ? 0 : smod + 1;   // s % H
-    for (int i = node0 + tid / T; i < node1; i += nthr / T) b_item(i, cbuf, k, s, buf, wslot, false);
+    for (int i = node0 + tid / T; i < node1; i += nthr / T) b_item(i, cbuf, k, s, kc.si[buf], wslot, false);
     // bookkeeping of step k+1 into the other buffer (read after the next
     // phase A barrier; its previous readers finished before this step's)
     if (tid == 0)
@@ -906,7 +949,7 @@ __global__ void __launch_bounds__(1024) k_dense(StepArgs a) {
         for (int u = 0; u < NCP; ++u) cpv[u] = lane_param<R>(a, par, pst, NDP + u);
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch) c[ch] = C::post((R)d[ch][v], cpv);
-        update_node<R, M, CF, RH>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s, buf, par, pst, c, bad,
+        update_node<R, M, CF, RH>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s, kc.si[buf], par, pst, c, bad,
                                   [&](int ch, R val) { Xn[((int64_t)ch * KP + i) * T + t] = (RH)val; });
       }
     }
@@ -975,7 +1018,11 @@ hipError_t launch_step_t(const StepArgs& a, int ntiles, int nthreads, size_t lds
   const bool pipe = a.uniform_rows && !a.zero_delay && Cfun<M, CF>::NCH * a.T <= 64 && !a.stim_tp && !a.has_aff;
   const uint64_t mmask = M::NDP >= 64 ? ~0ull : ((1ull << M::NDP) - 1);
   const bool um = pipe && (a.par_umask & mmask) == mmask && !a.spatial;
-  auto kern = !pipe ? k_step<RH, R, M, CF, 0, false> : um ? k_step<RH, R, M, CF, 1, true> : k_step<RH, R, M, CF, 1, false>;
+  // the time-blocked sequential loop: blocks of tb_k <= d_min steps, no
+  // afferent monitor, no zero delay
+  const bool tb = !pipe && a.tb_k > 1 && !a.has_aff && !a.zero_delay;
+  auto kern = pipe ? (um ? k_step<RH, R, M, CF, 1, true> : k_step<RH, R, M, CF, 1, false>)
+                   : tb ? k_step<RH, R, M, CF, 2, false> : k_step<RH, R, M, CF, 0, false>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;

@@ -335,6 +335,12 @@ class Plan:
         self._chk(self._lib.tvbk_plan_tiles(self.handle, C.byref(n), C.byref(k)))
         return n.value, k.value
 
+    def time_block(self):
+        """Steps per block of the time-blocked sequential loop (0: off)."""
+        k = C.c_int32()
+        self._chk(self._lib.tvbk_plan_time_block(self.handle, C.byref(k)))
+        return k.value
+
     def flags(self):
         f = np.zeros(self.n_inst, np.int32)
         self._chk(self._lib.tvbk_read_flags(self.handle, L.ptr(f)))

@@ -655,11 +655,12 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
       total += cost[b];
     }
     // aim at `div` tiles of equal per-step cost per CU (TVBK_SPLIT_DIV,
-    // default 1: finer splits cost more in per-block launch work and idle
-    // lanes than they gain in balance)
+    // default 4), parts of at least 512 edges and 8 nodes: with the
+    // time-blocked loop a part gathers a whole block at once, so finer parts
+    // keep more CUs busy (at 16 per CU the per-block work no longer pays)
     const char* envd = std::getenv("TVBK_SPLIT_DIV");
-    const double div = std::max(0.05, envd ? std::atof(envd) : 1.0);
-    const int64_t target = std::max<int64_t>((int64_t)((double)total / (div * std::max(n_cu, 1))), 2048);
+    const double div = std::max(0.05, envd ? std::atof(envd) : 4.0);
+    const int64_t target = std::max<int64_t>((int64_t)((double)total / (div * std::max(n_cu, 1))), 512);
     int kmin = 1 << 30;
     std::vector<TileDesc> extra;
     for (int k = 0; k < ntiles && forced != 0; ++k) {
@@ -668,7 +669,7 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
       const tvbk_connectome& cn = d.conns[td.conn];
       if (cd.d_min < 4) continue;                      // blocks of at least four steps
       int parts = forced > 1 ? forced : (int)((cost[td.inst0] + target - 1) / target);
-      parts = std::min(parts, std::min(32, cd.n_node / 16));
+      parts = std::min(parts, std::min(64, cd.n_node / 8));
       if (parts < 2) continue;
       // node boundaries at equal shares of sum(row length + 1)
       std::vector<int> bound(parts + 1, cd.n_node);

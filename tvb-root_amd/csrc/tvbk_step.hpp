@@ -280,12 +280,14 @@ __device__ __forceinline__ void monitors_node(const StepArgs& a, const KCfg& kc,
 // Integrator step, history write and monitors of one (node i, lane t) pair
 // with its coupling terms c (phase B of the step kernels).  `sink(ch, v)`
 // stores history channel ch of the new state.  EXT: region stimulus and
-// afferent monitors may be present (the sequential loop of k_step).
+// afferent monitors may be present (the sequential loops of k_step).
+// zpre: this step's injected normals when the caller has loaded them
+// already (else they are loaded here).
 template <class R, class M, int CF, class NR, bool UM = false, bool EXT = false, class SINK>
 __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, unsigned char* smem, R* s_x, int i,
                                             int t, int T, int N, bool live, int inst, uint64_t stream, int k,
                                             int64_t s, const StepInfo* sir, const R* par, int pst, const R* c,
-                                            int& bad, SINK sink) {
+                                            const double* zpre, int& bad, SINK sink) {
   using C = Cfun<M, CF>;
   constexpr int NSV = M::NSV, NDP = M::NDP, NCP = C::NCP, NCH = C::NCH;
   // this tile's descriptor: read-only during the kernel, scalar loads
@@ -317,7 +319,7 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
     // padded lanes (t >= n_inst) read lane 0's noise: never past the array
     const double* z = a.z + td.z_off + k * td.z_sk + (int64_t)i * td.z_si + (live ? t : 0);
 #pragma unroll
-    for (int v = 0; v < NSV; ++v) xi[v] = (R)((a.sqrt_dt * z[v * td.z_ssv]) * (double)gf[v]);
+    for (int v = 0; v < NSV; ++v) xi[v] = (R)((a.sqrt_dt * (zpre ? zpre[v] : z[v * td.z_ssv])) * (double)gf[v]);
   } else if (a.noise == NOISE_PHILOX) {
     // normals in the history precision NR (float for the float32-history
     // modes), the increment in the state precision
@@ -681,7 +683,8 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   // With `stage`, the new history channels go to this item's entries of the
   // coupling buffer it has just read (NCT == NCH) instead of the ring, and
   // ring_out() stores them later.
-  auto b_item = [&](int i, RH* cbuf, int k, int64_t s_, const StepInfo* sir, int wslot, bool stage) {
+  auto b_item = [&](int i, RH* cbuf, int k, int64_t s_, const StepInfo* sir, int wslot, bool stage,
+                    const double* zpre) {
     R c[NCT];
     R cpv[NCP > 0 ? NCP : 1];
 #pragma unroll
@@ -692,7 +695,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
                     : ring + (wslot * N + i) * rstride + t;
     RH* st = cbuf + (i * NCT) * T + t;
     update_node<R, M, CF, RH, UM, PIPE != 1>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s_, sir, par,
-                                             pst, c, bad,
+                                             pst, c, zpre, bad,
                                   [&](int ch, R v) {
                                     if (stage) st[ch * T] = (RH)v;
                                     else wr[ch * T] = (RH)v;
@@ -739,7 +742,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
       auto body = [&](int u, RH* vf, int ef, int nf, RH* vi, int& ei, int& ni) {
         if (u + 1 < nrow) ni = row_issue(RPc{}, w0 + (u + 1) * nwave, base_n, lane_off, vi, ei);
         const int ib = tid / T + u * (nthr / T);
-        if (ib < N) b_item(ib, cbuf, k, s, kc.si[buf], wslot, NCT == NCH);
+        if (ib < N) b_item(ib, cbuf, k, s, kc.si[buf], wslot, NCT == NCH, nullptr);
         if (u < nrow) row_store(cnext, w0 + u * nwave, lane, row_finish(RPc{}, ef, nf, base_n, lane_off, vf));
       };
       for (int u = 0; u < nu; u += 2) {
@@ -769,11 +772,28 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
       else if (a.csr_in_lds) phase_a(rec_lds, l_rp, s_c, nk);
       else phase_a(rec_glb, g_rp, s_c, nk);
       __syncthreads();
+      // injected normals (models of up to four state variables): the next
+      // step's are loaded before the current step is integrated, so their
+      // latency hides behind the update
+      constexpr bool ZPF = NSV <= 4;
+      const bool zpf = ZPF && a.noise == NOISE_INJECTED;
       for (int i = node0 + tid / T; i < node1; i += nthr / T) {
+        double zn[ZPF ? NSV : 1] = {}, zc[ZPF ? NSV : 1];
+        auto zload = [&](int k) {
+          if constexpr (ZPF) {
+            const double* z = a.z + td.z_off + (int64_t)k * td.z_sk + (int64_t)i * td.z_si + (live ? t : 0);
+#pragma unroll
+            for (int v = 0; v < NSV; ++v) zn[v] = z[v * td.z_ssv];
+          }
+        };
+        if (zpf) zload(kb);
         int ws = smod;
         for (int j = 0; j < nk; ++j) {
+#pragma unroll
+          for (int v = 0; v < (ZPF ? NSV : 1); ++v) zc[v] = zn[v];
+          if (zpf && j + 1 < nk) zload(kb + j + 1);
           ws = (ws + 1 == H) ? 0 : ws + 1;                // (s + j) % H
-          b_item(i, s_c + j * cstep, kb + j, s + j, si_blk + j * TVBK_KMAX_MON, ws, false);
+          b_item(i, s_c + j * cstep, kb + j, s + j, si_blk + j * TVBK_KMAX_MON, ws, false, zpf ? zc : nullptr);
         }
       }
       s += nk;
@@ -810,7 +830,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     }
     // ================= phase B: integrate, write history, monitors =========
     const int wslot = (smod + 1 == H) ? 0 : smod + 1;   // s % H
-    for (int i = node0 + tid / T; i < node1; i += nthr / T) b_item(i, cbuf, k, s, kc.si[buf], wslot, false);
+    for (int i = node0 + tid / T; i < node1; i += nthr / T) b_item(i, cbuf, k, s, kc.si[buf], wslot, false, nullptr);
     // bookkeeping of step k+1 into the other buffer (read after the next
     // phase A barrier; its previous readers finished before this step's)
     if (tid == 0)
@@ -949,7 +969,8 @@ __global__ void __launch_bounds__(1024) k_dense(StepArgs a) {
         for (int u = 0; u < NCP; ++u) cpv[u] = lane_param<R>(a, par, pst, NDP + u);
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch) c[ch] = C::post((R)d[ch][v], cpv);
-        update_node<R, M, CF, RH>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s, kc.si[buf], par, pst, c, bad,
+        update_node<R, M, CF, RH>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s, kc.si[buf], par, pst, c,
+                                  nullptr, bad,
                                   [&](int ch, R val) { Xn[((int64_t)ch * KP + i) * T + t] = (RH)val; });
       }
     }

@@ -120,6 +120,7 @@ struct tvbk_plan {
   std::string err;
   int max_n = 0;
   int64_t max_nnz = 0;
+  int64_t csr_nnz = 0;   // edges a tile stages in LDS: the largest tile's rows
   int spatial = 0;
   // tiles
   std::vector<TileDesc> tiles;
@@ -298,7 +299,7 @@ static int layout_lds(tvbk_plan* p, int T, int64_t max_lds, bool allow_csr) {
   p->lds_si = (int)off;
   if (p->tb_k > 1) off = align_up(off + (int64_t)p->tb_k * TVBK_KMAX_MON * TVBK_STEPINFO_BYTES, 16);
   // CSR staged as 16-byte records {col offset, delay, weight bits}
-  const int64_t ne = p->max_nnz + TVBK_ROW_PAD;
+  const int64_t ne = p->csr_nnz + TVBK_ROW_PAD;
   const int64_t csr = ne * 16 + align_up((N + 1) * 4, 16);
   p->csr_in_lds = allow_csr && (off + csr <= max_lds);
   if (p->csr_in_lds) {
@@ -397,6 +398,7 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
     }
     p->max_n = std::max(p->max_n, cn.n_node);
     p->max_nnz = std::max<int64_t>(p->max_nnz, cn.nnz);
+    p->csr_nnz = p->max_nnz;
     p->conns.push_back(cd);
   }
   // padding past the last edge: rows are read in fixed-width blocks
@@ -534,29 +536,6 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   // scalar cache (no CSR in LDS)
   p->uniform_rows = kernel == TVBK_KERNEL_BATCH_LANE && (p->nch * T) % 32 == 0;
   if (p->uniform_rows) layout_lds(p, T, max_lds, false);
-  // time-blocked sequential loop: plans that cannot take the pipelined loop,
-  // with every delay >= 2 and no afferent monitor, gather the coupling of
-  // up to TVBK_TB_MAX (<= d_min) steps at once -- as many coupling buffers
-  // as LDS holds without moving the CSR out of it.  TVBK_TIME_BLOCK: unset =
-  // automatic, 0 or 1 = off, n = blocks of at most n steps
-  p->tb_k = 0;
-  int tb_cap = TVBK_TB_MAX;
-  if (const char* env = std::getenv("TVBK_TIME_BLOCK"))
-    if (*env) tb_cap = std::max(0, std::min(TVBK_TB_MAX, std::atoi(env)));
-  if (kernel != TVBK_KERNEL_DENSE_MFMA && !(p->uniform_rows && p->nch * T <= 64) && !any_aff && tb_cap >= 2) {
-    int dmin = tb_cap;
-    for (const ConnDesc& cd : p->conns)
-      if (cd.nnz > 0) dmin = std::min(dmin, cd.d_min);
-    if (dmin >= 2) {
-      const bool csr0 = p->csr_in_lds, allow = !p->uniform_rows;
-      for (int K = dmin; K >= 2; --K) {
-        p->tb_k = K;
-        if (layout_lds(p, T, max_lds, allow) == TVBK_OK && p->csr_in_lds == csr0) break;
-        p->tb_k = 0;
-      }
-      if (!p->tb_k) layout_lds(p, T, max_lds, allow);
-    }
-  }
   {
     const int P = p->max_n * std::max(p->nch, 1);
     int nw;
@@ -716,6 +695,42 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
     } else {
       stage_len = 0;
       for (TileDesc& td : p->tiles) { td.parts = 1; td.node0 = 0; td.node1 = p->conns[td.conn].n_node; }
+    }
+  }
+  // a tile stages only its own rows of the CSR in LDS: with split instances
+  // the largest such range may fit where whole connectomes did not
+  if (kernel == TVBK_KERNEL_EDGE_LANE) {
+    int64_t rn = 0;
+    for (const TileDesc& td : p->tiles) {
+      const tvbk_connectome& cn = d.conns[td.conn];
+      rn = std::max<int64_t>(rn, (int64_t)cn.row_ptr[td.node1] - cn.row_ptr[td.node0]);
+    }
+    if (rn < p->csr_nnz) {
+      p->csr_nnz = rn;
+      layout_lds(p, 1, max_lds, true);
+    }
+  }
+  // time-blocked sequential loop: plans that cannot take the pipelined loop,
+  // with every delay >= 2 and no afferent monitor, gather the coupling of
+  // up to TVBK_TB_MAX (<= d_min) steps at once -- as many coupling buffers
+  // as LDS holds without moving the CSR out of it.  TVBK_TIME_BLOCK: unset =
+  // automatic, 0 or 1 = off, n = blocks of at most n steps
+  p->tb_k = 0;
+  int tb_cap = TVBK_TB_MAX;
+  if (const char* env = std::getenv("TVBK_TIME_BLOCK"))
+    if (*env) tb_cap = std::max(0, std::min(TVBK_TB_MAX, std::atoi(env)));
+  if (kernel != TVBK_KERNEL_DENSE_MFMA && !(p->uniform_rows && p->nch * T <= 64) && !any_aff && tb_cap >= 2) {
+    int dmin = tb_cap;
+    for (const ConnDesc& cd : p->conns)
+      if (cd.nnz > 0) dmin = std::min(dmin, cd.d_min);
+    if (dmin >= 2) {
+      const bool csr0 = p->csr_in_lds, allow = !p->uniform_rows;
+      for (int K = dmin; K >= 2; --K) {
+        p->tb_k = K;
+        if (layout_lds(p, T, max_lds, allow) == TVBK_OK && p->csr_in_lds == csr0) break;
+        p->tb_k = 0;
+      }
+      if (!p->tb_k) layout_lds(p, T, max_lds, allow);
     }
   }
 

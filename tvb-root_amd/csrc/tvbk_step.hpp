@@ -379,7 +379,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   const int conn = tdg->conn;
   const ConnDesc* cdg = a.conns + conn;
   const int N = cdg->n_node, H = cdg->horizon;
-  const int cd_nnz = cdg->nnz, cd_max_row = cdg->max_row;
+  const int cd_max_row = cdg->max_row;
   const int64_t cd_e_off = cdg->e_off, cd_rp_off = cdg->rp_off;
   const int td_inst0 = tdg->inst0, td_n_inst = tdg->n_inst;
   const int64_t td_step0 = tdg->step0, td_n_steps = tdg->n_steps, td_ring_off = tdg->ring_off,
@@ -427,17 +427,21 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   const int32_t* g_rp = a.row_ptr + cd_rp_off;
   EdgeRec* l_rec = reinterpret_cast<EdgeRec*>(smem + a.lds_edge);
   int32_t* l_rp = reinterpret_cast<int32_t*>(smem + a.lds_rp);
+  // the tile's rows only (a split tile holds a node range): records of
+  // edges ebase .. of its rows plus the padding
+  const int ebase = a.csr_in_lds ? g_rp[node0] : 0;
   if (a.csr_in_lds) {
-    for (int k = tid; k < cd_nnz + TVBK_ROW_PAD; k += nthr) {
-      const uint32_t ed = g_edge[k];
+    const int ecnt = g_rp[node1] - ebase + TVBK_ROW_PAD;
+    for (int k = tid; k < ecnt; k += nthr) {
+      const uint32_t ed = g_edge[ebase + k];
       EdgeRec r;
       r.coff = (ed & 0xffffu) * cstride;
       r.delay = ed >> 16;
       if constexpr (sizeof(RH) == 4) {
-        r.w0 = __float_as_uint((float)g_w[k]);
+        r.w0 = __float_as_uint((float)g_w[ebase + k]);
         r.w1 = 0;
       } else {
-        const double wd = (double)g_w[k];
+        const double wd = (double)g_w[ebase + k];
         r.w0 = (uint32_t)__double2loint(wd);
         r.w1 = (uint32_t)__double2hiint(wd);
       }
@@ -662,7 +666,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
       }
     }
   };
-  auto rec_lds = [&](int e) -> EdgeRec { return l_rec[e]; };
+  auto rec_lds = [&](int e) -> EdgeRec { return l_rec[e - ebase]; };
   auto rec_glb = [&](int e) -> EdgeRec {
     const uint32_t ed = g_edge[e];
     EdgeRec r;

@@ -769,9 +769,11 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     StepInfo* si_blk = reinterpret_cast<StepInfo*>(smem + a.lds_si);
     for (int kb = kbeg; kb < kfin; kb += a.tb_k) {
       const int nk = kfin - kb < a.tb_k ? kfin - kb : a.tb_k;
-      if (tid == 0)
-        for (int j = 0; j < nk; ++j)
-          for (int m = 0; m < a.n_mon; ++m) step_info(kc.mon[m], kb + j, si_blk[j * TVBK_KMAX_MON + m]);
+      // one thread per (step, monitor) record
+      for (int q = tid; q < nk * a.n_mon; q += nthr) {
+        const int j = q / a.n_mon, m = q - j * a.n_mon;
+        step_info(kc.mon[m], kb + j, si_blk[j * TVBK_KMAX_MON + m]);
+      }
       if (a.uniform_rows) phase_a_uni(s_c, nk);
       else if (a.csr_in_lds) phase_a(rec_lds, l_rp, s_c, nk);
       else phase_a(rec_glb, g_rp, s_c, nk);

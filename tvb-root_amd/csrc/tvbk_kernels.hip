@@ -23,7 +23,7 @@ __global__ void k_scatter(bool gather, RH* ring, R* st, RH* hist, R* state, int 
        q += (int64_t)gridDim.x * blockDim.x) {
     if (q < nh) {
       const int i = (int)(q % N), ch = (int)((q / N) % nch), slot = (int)(q / ((int64_t)N * nch));
-      RH* r = ring + (((int64_t)slot * N + i) * nch + ch) * T + t;
+      RH* r = ring + ring_index(T, H, N, nch, slot, i, ch, t);
       if (gather) hist[q] = *r; else *r = hist[q];
     } else {
       const int64_t u = q - nh;
@@ -206,8 +206,9 @@ __global__ void k_commit(const TileDesc* __restrict__ tiles, const ConnDesc* __r
     const int j = (int)(q / per);
     const int64_t r = q - (int64_t)j * per;
     const int i = td.node0 + (int)(r / rs), e = (int)(r % rs);
+    const int ch = e / T, t = e - (e / T) * T;
     const int slot = (int)((td.step0 + k0 + j) % H);
-    ring[td.ring_off + ((int64_t)slot * N + i) * rs + e] = stage[td.stage_off + ((int64_t)j * N + i) * rs + e];
+    ring[td.ring_off + ring_index(T, H, N, nch, slot, i, ch, t)] = stage[td.stage_off + ((int64_t)j * N + i) * rs + e];
   }
 }
 

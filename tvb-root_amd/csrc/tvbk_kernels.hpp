@@ -23,6 +23,16 @@ struct ConnDesc {
 #define TVBK_TB_MAX 32
 #define TVBK_STEPINFO_BYTES 48
 
+// Element index of history value (slot, node i, channel ch, lane t) in a
+// tile's ring.  Tiles of T > 1 instances keep a slot's values together
+// ([slot][node][channel][T]: a wave's gather of one edge reads one run of
+// channels x instances); single-instance tiles (T = 1) keep a node's slots
+// together ([node][channel][slot]: consecutive steps of one edge read
+// consecutive values, so a time block's gathers of an edge share lines).
+__host__ __device__ inline int64_t ring_index(int T, int H, int N, int nch, int slot, int i, int ch, int t) {
+  return T == 1 ? ((int64_t)i * nch + ch) * H + slot : (((int64_t)slot * N + i) * nch + ch) * T + t;
+}
+
 // One workgroup's tile of instances and where its data lives.
 struct TileDesc {
   int32_t conn, n_inst, inst0;

@@ -22,7 +22,8 @@
 // the history write before the next gather.
 //
 // Layouts (elements of the plan precision R unless stated):
-//   history  ring[tile][slot][node][channel][T]
+//   history  ring[tile][slot][node][channel][T]; single-instance tiles
+//            ring[tile][node][channel][slot] (ring_index)
 //   LDS      bold interim (f64) | state [node][svar][T] | tavg sum, comp
 //            [node][voi][T] | c[2][node][term][T] | CSR (packed col|delay,
 //            weight, row_ptr) when it fits
@@ -396,6 +397,13 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   const int rstride = NCH * T;                                   // elements per (slot, node)
   const uint32_t cstride = (uint32_t)rstride * (uint32_t)sizeof(RH);  // bytes per node in a slot
   const uint32_t sstride = (uint32_t)N * cstride;                // bytes per slot (< 2^24)
+  // byte strides of the gather (ring_index): slot, source node, channel;
+  // single-instance tiles keep a node's slots together
+  const bool tmaj = T == 1;
+  const uint32_t g_sl = tmaj ? (uint32_t)sizeof(RH) : sstride;
+  const uint32_t g_nd = tmaj ? (uint32_t)(NCH * H) * (uint32_t)sizeof(RH) : cstride;
+  const uint32_t g_ch = tmaj ? (uint32_t)H * (uint32_t)sizeof(RH) : (uint32_t)T * (uint32_t)sizeof(RH);
+  const int64_t w_ch = tmaj ? H : T;                             // elements between channels
 
   if (tid == 0) {
     kc.td = *tdg;
@@ -435,7 +443,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     for (int k = tid; k < ecnt; k += nthr) {
       const uint32_t ed = g_edge[ebase + k];
       EdgeRec r;
-      r.coff = (ed & 0xffffu) * cstride;
+      r.coff = (ed & 0xffffu) * g_nd;
       r.delay = ed >> 16;
       if constexpr (sizeof(RH) == 4) {
         r.w0 = __float_as_uint((float)g_w[ebase + k]);
@@ -499,23 +507,25 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   auto slot_off = [&](int sm, uint32_t delay) -> uint32_t {
     int sl = sm - (int)delay;
     sl += (sl < 0) ? H : 0;
-    return __umul24((uint32_t)sl, sstride);
+    return __umul24((uint32_t)sl, g_sl);
   };
   // phase A of nk consecutive steps, the first with (s - 1) % H = smod
   // (nk <= d_min: every slot read was written before the first of them):
-  // items (step j, node-channel pair q), step j's sums into cbuf0 + j * cstep
+  // items (node-channel pair q, step j), row-major so that the lanes of a
+  // row read consecutive slots of each edge; step j's sums into
+  // cbuf0 + j * cstep
   const int64_t cstep = (int64_t)N * NCT * T;
   auto phase_a = [&](auto getrec, const int32_t* RP, RH* cbuf0, int nk) {
     const int q0 = node0 * NCH, P = (node1 - node0) * NCH, tot = nk * P;
     for (int it = wave * NG + g; it - g < tot; it += nwave * NG) {
       if (it < tot) {
-        const int j = it / P, q = q0 + (it - j * P);
+        const int qq = it / nk, j = it - qq * nk, q = q0 + qq;
         const int sm = smod + j < H ? smod + j : smod + j - H;
         RH* cbuf = cbuf0 + j * cstep;
         const int i = q / NCH, ch = q - (q / NCH) * NCH;
         const int e0 = RP[i];
         const int n = RP[i + 1] - e0;
-        const uint32_t lane_off = (uint32_t)(ch * T + t) * (uint32_t)sizeof(RH);
+        const uint32_t lane_off = (uint32_t)ch * g_ch + (uint32_t)t * (uint32_t)sizeof(RH);
         RH acc;
         // one path per wave (both give the same sums): a wave whose rows
         // are not all short takes the blocked loads for every row
@@ -670,7 +680,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   auto rec_glb = [&](int e) -> EdgeRec {
     const uint32_t ed = g_edge[e];
     EdgeRec r;
-    r.coff = (ed & 0xffffu) * cstride;
+    r.coff = (ed & 0xffffu) * g_nd;
     r.delay = ed >> 16;
     if constexpr (sizeof(RH) == 4) {
       r.w0 = __float_as_uint((float)g_w[e]);
@@ -696,13 +706,14 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
 #pragma unroll
     for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(i * NCT + v) * T + t], cpv);
     RH* wr = staged ? reinterpret_cast<RH*>(a.stage) + stage_off + ((int64_t)(k - kbeg) * N + i) * rstride + t
-                    : ring + (wslot * N + i) * rstride + t;
+                    : ring + ring_index(T, H, N, NCH, wslot, i, 0, t);
+    const int64_t wch = staged ? T : w_ch;
     RH* st = cbuf + (i * NCT) * T + t;
     update_node<R, M, CF, RH, UM, PIPE != 1>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s_, sir, par,
                                              pst, c, zpre, bad,
                                   [&](int ch, R v) {
                                     if (stage) st[ch * T] = (RH)v;
-                                    else wr[ch * T] = (RH)v;
+                                    else wr[ch * wch] = (RH)v;
                                   });
   };
   // store this thread's staged history channels of the step into slot wslot
@@ -1029,7 +1040,7 @@ __global__ void k_init_uniform(InitArgs a) {
     const R* prow = reinterpret_cast<const R*>(a.params) + (int64_t)inst * a.prow;
     R cp[C::NCP > 0 ? C::NCP : 1];
     for (int k = 0; k < C::NCP; ++k) cp[k] = prow[NDP + k];
-    for (int ch = 0; ch < NCH; ++ch) ring[((int64_t)slot * N + i) * NCH * T + ch * T + t] = (RH)C::template channel<R>(x, ch, cp);
+    for (int ch = 0; ch < NCH; ++ch) ring[ring_index(T, H, N, NCH, slot, i, ch, t)] = (RH)C::template channel<R>(x, ch, cp);
     if (slot == 0)
       for (int v = 0; v < NSV; ++v) st[(i * NSV + v) * T + t] = x[v];
   }

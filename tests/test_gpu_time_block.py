@@ -83,6 +83,31 @@ def test_single_instance_blocked_equals_stepwise_and_oracle(precision):
             assert rel_err(y, yr) <= 1e-10, (m, rel_err(y, yr))
 
 
+def test_area_monitors_blocked_match_oracle():
+    """GlobalAverage and SpatialAverage (node samples reduced after the
+    launch) through the time-blocked loop: bit-exact against the oracle's
+    reference-faithful mode (SpatialAverage to 1e-14: the reference's dot
+    product runs through BLAS)."""
+    from tvb_hip.backend import HipBackend
+    _say("time block area monitors")
+    n = 24
+    mons = (("GlobalAverage", 0.5, None), ("SpatialAverage", 1.0, None, dict(spatial_mask=np.arange(n) % 3)),
+            ("TemporalAverage", 1.0, None))
+    orc, sim, _ = make_case("MontbrioPazoRoxin", n=n, density=0.3, lengths=(10.0, 60.0), speed=3.0, dt=0.05,
+                            scheme="HeunStochastic", cparams=dict(a=0.2), nsig=[0.01, 0.02], seed=41, monitors=mons)
+    assert _block_of(sim, precision="mixed") > 1
+    o = orc["faithful"]
+    z = o.legacy_noise(900)
+    ref = o.run(900, noise=z)
+    out = HipBackend().run_sim(sim, nstep=900, precision="mixed", noise=z, chunk=400)
+    for m, ((t, y), (tr, yr)) in enumerate(zip(out, ref)):
+        np.testing.assert_allclose(t, tr, rtol=0, atol=1e-9)
+        if m == 1:
+            assert rel_err(y, yr) <= 1e-14
+        else:
+            np.testing.assert_array_equal(y, yr, err_msg="monitor %d" % m)
+
+
 def test_short_delays_give_short_blocks():
     """Blocks are at most d_min steps long; d_min = 1 turns the blocking off."""
     _say("time block short delays")

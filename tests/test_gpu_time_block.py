@@ -194,3 +194,27 @@ def test_batch_lane_groups_blocked_equals_stepwise():
             assert q1 == q0
             np.testing.assert_array_equal(h1, h0)
             np.testing.assert_array_equal(x1, x0)
+
+
+@pytest.mark.parametrize("case", ["one-node", "no-edges"])
+def test_connectome_without_edges(case):
+    """A one-node simulator and a connectome of zero weights (horizon 1):
+    no time blocks, and the oracle's trajectory bit for bit."""
+    from tvb_hip.backend import HipBackend
+    _say("no edges", case)
+    if case == "one-node":
+        orc, sim, _ = make_case("MontbrioPazoRoxin", n=1, dt=0.05, scheme="HeunStochastic", nsig=[0.01, 0.02],
+                                seed=3, monitors=(("TemporalAverage", 1.0, None), ("Raw", None, None)))
+    else:
+        w = np.zeros((6, 6))
+        orc, sim, _ = make_case("Generic2dOscillator", weights=w, tract_lengths=w.copy(), dt=0.1, seed=3,
+                                monitors=(("Raw", None, None), ("TemporalAverage", 0.5, None)))
+    assert sim.connectivity.horizon == 1
+    assert _block_of(sim, precision="mixed") == 0
+    o = orc["faithful"]
+    z = o.legacy_noise(300) if case == "one-node" else None
+    ref = o.run(300, noise=z)
+    out = HipBackend().run_sim(sim, nstep=300, precision="mixed", noise=z if z is not None else "legacy", chunk=120)
+    for m, ((t, y), (tr, yr)) in enumerate(zip(out, ref)):
+        np.testing.assert_allclose(t, tr, rtol=0, atol=1e-9)
+        np.testing.assert_array_equal(y, yr, err_msg="monitor %d" % m)

@@ -720,9 +720,10 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   if (const char* env = std::getenv("TVBK_TIME_BLOCK"))
     if (*env) tb_cap = std::max(0, std::min(TVBK_TB_MAX, std::atoi(env)));
   if (kernel != TVBK_KERNEL_DENSE_MFMA && !(p->uniform_rows && p->nch * T <= 64) && !any_aff && tb_cap >= 2) {
+    // (a connectome without edges reads nothing; its blocks stay below H
+    // so that the slot counters wrap once per block)
     int dmin = tb_cap;
-    for (const ConnDesc& cd : p->conns)
-      if (cd.nnz > 0) dmin = std::min(dmin, cd.d_min);
+    for (const ConnDesc& cd : p->conns) dmin = std::min(dmin, cd.nnz > 0 ? cd.d_min : cd.horizon - 1);
     if (dmin >= 2) {
       const bool csr0 = p->csr_in_lds, allow = !p->uniform_rows;
       for (int K = dmin; K >= 2; --K) {

@@ -778,8 +778,10 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     // and coupling).  Two barriers per block; the monitors' step records of
     // the block are kept in LDS.
     StepInfo* si_blk = reinterpret_cast<StepInfo*>(smem + a.lds_si);
-    for (int kb = kbeg; kb < kfin; kb += a.tb_k) {
-      const int nk = kfin - kb < a.tb_k ? kfin - kb : a.tb_k;
+    // blocks never exceed H steps (the plan keeps them <= d_min < H)
+    const int KT = a.tb_k < H ? a.tb_k : H;
+    for (int kb = kbeg; kb < kfin; kb += KT) {
+      const int nk = kfin - kb < KT ? kfin - kb : KT;
       // one thread per (step, monitor) record
       for (int q = tid; q < nk * a.n_mon; q += nthr) {
         const int j = q / a.n_mon, m = q - j * a.n_mon;

@@ -195,45 +195,50 @@ __device__ __forceinline__ void monitors_node(const StepArgs& a, const KCfg& kc,
                (int64_t)i * td.out_si[m] + t;
       };
       R* ts = reinterpret_cast<R*>(smem + mo.lds_off);
-      for (int v = 0; v < nv; ++v) {
-        const R val = sample_val(mo, v);
-        if (mo.kind == MON_RAW) {
-          if (live && emit) outp()[v * td.out_sv[m]] = val;
+      // running sum over the averaging window: Kahan-compensated in float32
+      // (sum su, compensation cm), plain sequential (numpy's mean order) in
+      // float64 (sum only); su and cm are the LDS values of value v
+      constexpr int NSL = sizeof(R) == 8 ? 1 : 2;
+      auto tavg_value = [&](int v, R val, R& su, R& cm) {
+        R tv;
+        if constexpr (sizeof(R) == 8) {
+          tv = su + val;
         } else {
-          // running sum over the averaging window: Kahan-compensated in
-          // float32 (sum, compensation), plain sequential (numpy's mean
-          // order) in float64 (sum only)
-          constexpr int NSL = sizeof(R) == 8 ? 1 : 2;
-          R* sum = ts + ((i * nv + v) * NSL + 0) * T + t;
-          R* cmp = ts + ((i * nv + v) * NSL + (NSL - 1)) * T + t;
-          R tv;
-          if constexpr (sizeof(R) == 8) {
-            tv = *sum + val;
-          } else {
-            const R yv = val - *cmp;
-            tv = *sum + yv;
-            *cmp = (tv - *sum) - yv;
-          }
-          *sum = tv;
-          if (emit) {
-            if (live) outp()[v * td.out_sv[m]] = tv / (R)mo.istep;
-            *sum = R(0);
-            if constexpr (sizeof(R) == 4) *cmp = R(0);
-          }
+          const R yv = val - cm;
+          tv = su + yv;
+          cm = (tv - su) - yv;
         }
+        su = tv;
+        if (emit) {
+          if (live) outp()[v * td.out_sv[m]] = tv / (R)mo.istep;
+          su = R(0);
+          cm = R(0);
+        }
+      };
+      auto slot_of = [&](int v, int q) -> R* { return ts + ((i * nv + v) * NSL + q) * T + t; };
+      int v0 = 0;
+      if (mo.kind == MON_RAW) {
+        for (int v = 0; v < nv; ++v)
+          if (live && emit) outp()[v * td.out_sv[m]] = sample_val(mo, v);
+        v0 = nv;
+      }
+      for (int v = v0; v < nv; ++v) {
+        R su = *slot_of(v, 0), cm = NSL == 2 ? *slot_of(v, NSL - 1) : R(0);
+        tavg_value(v, sample_val(mo, v), su, cm);
+        *slot_of(v, 0) = su;
+        if constexpr (NSL == 2) *slot_of(v, NSL - 1) = cm;
       }
     } else {  // Bold: interim mean every ii steps pushed into FIR accumulators
       double* bi = reinterpret_cast<double*>(smem + mo.lds_off);
       const bool push = si.push != 0;
       const int64_t mp = si.mp;                      // interim sample index s / ii
       double* acc = a.bacc + td.bacc_off[m];
-      for (int v = 0; v < nv; ++v) {
-        const R val = sample_val(mo, v);
-        double* bsum = bi + (i * nv + v) * T + t;
-        *bsum = *bsum + (double)val;
+      // interim sum bs (the LDS value of value v) and this step's sample
+      auto bold_value = [&](int v, R val, double& bs) {
+        bs = bs + (double)val;
         if (push) {
-          const double mean = *bsum / (double)mo.ii;
-          *bsum = 0.0;
+          const double mean = bs / (double)mo.ii;
+          bs = 0.0;
           if (mo.stock) {   // the reference's stock slot (mp - 1) % S (monitors.py:948), mp >= 1
             const int64_t sl = (mp - 1) % mo.S;
             mo.stock[td.bst_off[m] + ((sl * nv + v) * (int64_t)N + i) * T + t] = mean;
@@ -273,6 +278,12 @@ __device__ __forceinline__ void monitors_node(const StepArgs& a, const KCfg& kc,
           if (live) *out = bold;
           *ap = 0.0;
         }
+      };
+      auto bslot = [&](int v) -> double* { return bi + (i * nv + v) * T + t; };
+      for (int v = 0; v < nv; ++v) {
+        double bs = *bslot(v);
+        bold_value(v, sample_val(mo, v), bs);
+        *bslot(v) = bs;
       }
     }
   }

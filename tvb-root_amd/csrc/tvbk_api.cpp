@@ -634,11 +634,12 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
       total += cost[b];
     }
     // aim at `div` tiles of equal per-step cost per CU (TVBK_SPLIT_DIV,
-    // default 4), parts of at least 512 edges and 8 nodes: with the
+    // default 3), parts of at least 512 edges and 8 nodes: with the
     // time-blocked loop a part gathers a whole block at once, so finer parts
-    // keep more CUs busy (at 16 per CU the per-block work no longer pays)
+    // keep more CUs busy (measured on the uneven JansenRit batch: best at
+    // 2-3 per CU, slower at 8)
     const char* envd = std::getenv("TVBK_SPLIT_DIV");
-    const double div = std::max(0.05, envd ? std::atof(envd) : 4.0);
+    const double div = std::max(0.05, envd ? std::atof(envd) : 3.0);
     const int64_t target = std::max<int64_t>((int64_t)((double)total / (div * std::max(n_cu, 1))), 512);
     int kmin = 1 << 30;
     std::vector<TileDesc> extra;

@@ -218,3 +218,50 @@ def test_connectome_without_edges(case):
     for m, ((t, y), (tr, yr)) in enumerate(zip(out, ref)):
         np.testing.assert_allclose(t, tr, rtol=0, atol=1e-9)
         np.testing.assert_array_equal(y, yr, err_msg="monitor %d" % m)
+
+
+def test_uniform_rows_blocked_equals_stepwise_and_oracle():
+    """Tiles of 64 instances with two coupled variables (wave-uniform rows,
+    too wide for the pipelined loop): the time-blocked loop gathers a block's
+    rows for all its steps; bit-identical to stepping and, for the checked
+    instances, to the oracle (MIXED)."""
+    from tvb_hip import _lib as L
+    from tvb_hip.backend import HipBackend
+    _say("time block uniform rows")
+    cases = [make_case("MontbrioPazoRoxin", n=20, density=0.3, lengths=(10.0, 60.0), speed=3.0, dt=0.05,
+                       scheme="HeunStochastic", cparams=dict(a=0.1 + 0.005 * b), nsig=[0.01, 0.02], seed=8,
+                       monitors=(("TemporalAverage", 1.0, None), ("Bold", 20.0, None))) for b in range(80)]
+    sims = [c[1] for c in cases]
+    zs = [c[0]["faithful"].legacy_noise(700) for c in cases]
+    outs = []
+    for tb in (None, 0):
+        def run():
+            plan = HipBackend().build_batch_plan(sims, precision="mixed", noise="inject", max_steps=350,
+                                                 kernel="batch", tile=64)
+            assert plan.kernel == L.KERNEL_BATCH_LANE and plan.tile == 64
+            k = plan.time_block()
+            res = [[] for _ in plan.monitors]
+            for k0 in (0, 350):
+                plan.set_noise(np.stack([zz[k0:k0 + 350] for zz in zs], axis=-1))
+                plan.run(350)
+                for m in range(len(plan.monitors)):
+                    res[m].append(plan.read(m))
+            states = [plan.export(b) for b in (0, 70)]
+            plan.close()
+            return k, res, states
+        outs.append(_with_tb(tb, run))
+    (k1, r1, s1), (k0, r0, s0) = outs
+    assert k1 > 1 and k0 == 0, (k1, k0)
+    for m in range(len(r0)):
+        for c in range(2):
+            for b in range(80):
+                np.testing.assert_array_equal(r1[m][c][b][1], r0[m][c][b][1], err_msg="monitor %d inst %d" % (m, b))
+    for (h1, x1, q1), (h0, x0, q0) in zip(s1, s0):
+        np.testing.assert_array_equal(h1, h0)
+        np.testing.assert_array_equal(x1, x0)
+    for b in (0, 70):
+        ref = cases[b][0]["faithful"].run(700, noise=zs[b])
+        tavg = np.concatenate([r1[0][0][b][1], r1[0][1][b][1]])
+        np.testing.assert_array_equal(tavg[..., np.newaxis], ref[0][1], err_msg="inst %d" % b)
+        bold = np.concatenate([r1[1][0][b][1], r1[1][1][b][1]])
+        assert rel_err(bold[..., np.newaxis], ref[1][1]) <= 1e-13

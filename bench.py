@@ -1,23 +1,33 @@
 #!/usr/bin/env python
 """Benchmark of the TVB simulator loop on one MI355X.
 
-Workload (default, "c2"): batched parameter sweep on one shared 68-node
+Headline workload ("c2"): batched parameter sweep on one shared 68-node
 synthetic connectome (25 % density, log-normal weights, tract lengths
 U[10, 180] mm, speed 5 mm/ms -> delays 40..720 steps at dt 0.05 ms),
 8192 instances on a 128 x 64 grid of coupling gain G in [0, 2] (Linear a)
 x noise sigma in [1e-4, 1e-2] (Additive nsig of both state variables),
 MontbrioPazoRoxin, HeunStochastic dt = 0.05 ms, TemporalAverage(1 ms) +
-Bold(2000 ms), FP32 state / history / coupling, FP64 Bold, device Philox
-noise.  One "step" advances all 8192 x 68 region states by dt.
+Bold(2000 ms), device Philox noise, inputs resident in HBM.
 
-Prints one JSON line: region-steps/s of the batched FP32 sweep, the
-step-kernel roofline (algorithmic HBM bytes per launch over the launch
-time from HIP events), the numpy oracle timed on one host core on a sample
-of the same workload, the plain single-instance FP64 device path on the
-same connectome, and the sweep's TemporalAverage error against the FP64
-oracle on a sample.
+One bench "step" is one kernel launch that advances all 8192 x 68 region
+states by ``--launch-steps`` integration steps (default 20 000 = 1 s
+simulated, 1000 TemporalAverage samples, 250 Bold interim pushes; every
+second launch emits a Bold sample).  ``--steps 20`` times the sweep's full
+20 s.  ``--precision`` picks the arithmetic: "mixed" (default; float32
+history ring, weights and coupling sums, float64 state, integration and
+monitors -- the reference's own mix, the precision that meets the stated
+error bar) or "f32" (float32 state too).
+
+Prints ONE JSON line: the headline (region-steps/s, the step kernel's
+roofline from HIP events on the launch stream, the timed work), then the
+extras, each run under its own guard so that a failing extra is recorded
+in the line instead of losing it: the other precision, the plain
+single-instance FP64 device path, the error against the FP64 oracle, the
+gather-pattern ceiling, C1, C3, C4, the zero-delay MFMA variant, the
+Balloon-Windkessel pass and the CPU baselines.
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -32,24 +42,50 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_step_mpr_c2_hbm_traffic.md")
-
-
-def measured_traffic_per_region_step(path=TRAFFIC_SUMMARY):
-    """HBM bytes per region-step of the step kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (line
-    'traffic_bytes_per_region_step: X'), or None when there is none."""
-    try:
-        with open(path) as f:
-            for ln in f:
-                if ln.strip().startswith("traffic_bytes_per_region_step:"):
-                    return float(ln.split(":", 1)[1].split()[0])
-    except (OSError, ValueError):
-        pass
-    return None
+CSRC = os.path.join(ROOT, "tvb-root_amd", "csrc")
+ERROR_BAR = {"tavg_max_rel": 1e-4, "bold_max_rel": 1e-6}
+PREC_NAMES = ("f32", "f64", "mixed")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_source_sha16():
+    """Hash of the HIP library's sources (every file under csrc/): a traffic
+    profile taken on other sources does not describe the built kernel."""
+    h = hashlib.sha256()
+    for f in sorted(os.listdir(CSRC)):
+        with open(os.path.join(CSRC, f), "rb") as fh:
+            h.update(f.encode() + b"\0" + fh.read() + b"\0")
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(precision, channels, path=TRAFFIC_SUMMARY):
+    """HBM bytes per region-step of the step kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary, if that summary was taken on
+    the current sources for the same precision and history channels.
+    Returns (bytes per region-step or None, note)."""
+    keys = {}
+    try:
+        with open(path) as f:
+            for ln in f:
+                s = ln.strip().strip("`")
+                for k in ("traffic_bytes_per_region_step", "kernel_source_sha16", "precision", "channels"):
+                    if s.startswith(k + ":"):
+                        keys[k] = s.split(":", 1)[1].split()[0]
+    except OSError:
+        return None, "no traffic summary"
+    if keys.get("kernel_source_sha16") != kernel_source_sha16():
+        return None, "traffic summary was measured on other kernel sources (%s, built %s)" % (
+            keys.get("kernel_source_sha16"), kernel_source_sha16())
+    if keys.get("precision") != precision or keys.get("channels") != str(channels):
+        return None, "traffic summary is for precision %s / %s channels" % (keys.get("precision"),
+                                                                             keys.get("channels"))
+    try:
+        return float(keys["traffic_bytes_per_region_step"]), "rocprofv3 FETCH_SIZE+WRITE_SIZE, " + path
+    except (KeyError, ValueError):
+        return None, "traffic summary without a value"
 
 
 def c2_setup(B=8192, seed=2024):
@@ -75,7 +111,7 @@ def mpr_params(G, sig):
     return p
 
 
-def make_bold_spec(dt, period=2000.0):
+def make_bold_spec(dt, period=2000.0, voi=(0, 1)):
     from tvb_hip import _lib as L
     from tvb_hip.plan import MonitorSpec
     from tvb_hip.sim import Bold, iround
@@ -86,12 +122,12 @@ def make_bold_spec(dt, period=2000.0):
     b.compute_hrf()
     taps = b.hemodynamic_response_function[0, ::-1].copy()
     k1v0 = b.hrf_kernel.parameters["k_1"] * b.hrf_kernel.parameters["V_0"]
-    return MonitorSpec(L.MON_BOLD, b.istep, [0, 1], interim_istep=b._interim_istep, taps=taps,
+    return MonitorSpec(L.MON_BOLD, b.istep, list(voi), interim_istep=b._interim_istep, taps=taps,
                        k1_v0=k1v0, subtract_one=True, period=period, name="Bold")
 
 
 def build_c2_plan(w, d, G, sig, dt, max_steps, precision, noise, seed, tile=0, waves=0, tavg=1.0, bold=2000.0,
-                  stream=None):
+                  stream=None, kernel=0):
     from tvb_hip import _lib as L
     from tvb_hip.plan import CSR, MonitorSpec, Plan
     from tvb_hip.sim import iround
@@ -100,31 +136,20 @@ def build_c2_plan(w, d, G, sig, dt, max_steps, precision, noise, seed, tile=0, w
         mons.append(make_bold_spec(dt, bold))
     plan = Plan(model=L.MODEL_MPR, cfun=L.CFUN_LINEAR, scheme=L.HEUN_STO, precision=precision, noise=noise,
                 conns=[CSR(w, d)], params=mpr_params(G, sig), dt=dt, obs_coef=np.eye(2), monitors=mons,
-                max_steps=max_steps, seed=seed, tile=tile, waves=waves, stream=stream)
+                max_steps=max_steps, seed=seed, tile=tile, waves=waves, stream=stream, kernel=kernel)
     return plan
 
 
-def algorithmic_bytes_per_region_step(nnz, N, ncvar, s, nvoi_t, istep_t, nvoi_b, ii, S, istep_b, noise_bytes=0):
-    """Survey model: gather + ring write + noise + monitors."""
+def algorithmic_bytes_per_region_step(nnz, N, ncvar, s, nvoi_t, istep_t, nvoi_b, ii, S, istep_b, noise_bytes=0,
+                                      tavg_bytes=4):
+    """Gather (one history value per edge and channel) + ring write + noise
+    + monitors (TemporalAverage output, Bold interim push into the pending
+    HRF sums and their read at emission)."""
     b = s * ncvar * (nnz / N) + s * ncvar + noise_bytes
-    b += 4.0 * nvoi_t / istep_t if istep_t else 0.0
+    b += float(tavg_bytes) * nvoi_t / istep_t if istep_t else 0.0
     if istep_b:
         b += 8.0 * nvoi_b * (1.0 / ii + S / istep_b)
     return b
-
-
-def cpu_baseline(w, L, G, sig, n_steps):
-    """Reference-faithful numpy oracle, one thread, one instance of the sweep."""
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
-    from oracle.tvb_oracle import OracleSim
-    o = OracleSim(w, L, 5.0, 0.05, "MontbrioPazoRoxin", coupling_params=dict(a=float(G)),
-                  integrator="HeunStochastic", nsig=np.array([sig, sig]), initial_conditions=None,
-                  monitors=(("TemporalAverage", 1.0, None), ("Bold", 2000.0, None)), mode="faithful")
-    z = np.random.default_rng(0).standard_normal((n_steps, 2, w.shape[0]))
-    t0 = time.perf_counter()
-    o.run(n_steps, noise=z)
-    el = time.perf_counter() - t0
-    return w.shape[0] * n_steps / el, el
 
 
 def cpu_model():
@@ -136,6 +161,23 @@ def cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+# ---------------------------------------------------------------------------
+# CPU baselines: the reference-faithful numpy oracle on the host cores
+# ---------------------------------------------------------------------------
+
+def cpu_baseline(w, L, G, sig, n_steps):
+    """Reference-faithful numpy oracle, one thread, one instance of the sweep."""
+    from oracle.tvb_oracle import OracleSim
+    o = OracleSim(w, L, 5.0, 0.05, "MontbrioPazoRoxin", coupling_params=dict(a=float(G)),
+                  integrator="HeunStochastic", nsig=np.array([sig, sig]), initial_conditions=None,
+                  monitors=(("TemporalAverage", 1.0, None), ("Bold", 2000.0, None)), mode="faithful")
+    z = np.random.default_rng(0).standard_normal((n_steps, 2, w.shape[0]))
+    t0 = time.perf_counter()
+    o.run(n_steps, noise=z)
+    el = time.perf_counter() - t0
+    return w.shape[0] * n_steps / el, el
 
 
 _CPU_CHILD = r"""
@@ -154,6 +196,14 @@ print(time.perf_counter() - t0)
 """
 
 
+def host_procs():
+    return max(1, min(len(os.sched_getaffinity(0)), 16))
+
+
+def child_env():
+    return dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+
+
 def cpu_baseline_all_cores(w, L, n_steps, procs=None):
     """The oracle on every host core the run may use: ``procs`` concurrent
     single-threaded processes (at most 16), one independent instance of the
@@ -161,17 +211,14 @@ def cpu_baseline_all_cores(w, L, n_steps, procs=None):
     is all region-steps over the wall time of the slowest process."""
     import subprocess
     import tempfile
-    if procs is None:
-        procs = max(1, min(len(os.sched_getaffinity(0)), 16))
-    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    procs = procs or host_procs()
     with tempfile.TemporaryDirectory() as tmp:
         pw, pl = os.path.join(tmp, "w.npy"), os.path.join(tmp, "l.npy")
         np.save(pw, w)
         np.save(pl, L)
-        root = os.path.dirname(os.path.abspath(__file__))
         t0 = time.perf_counter()
-        ps = [subprocess.Popen([sys.executable, "-c", _CPU_CHILD, root, pw, pl, str(n_steps), str(k)],
-                               stdout=subprocess.PIPE, stderr=subprocess.PIPE, env=env, text=True)
+        ps = [subprocess.Popen([sys.executable, "-c", _CPU_CHILD, ROOT, pw, pl, str(n_steps), str(k)],
+                               stdout=subprocess.PIPE, stderr=subprocess.PIPE, env=child_env(), text=True)
               for k in range(procs)]
         outs = [p.communicate() for p in ps]
         wall = time.perf_counter() - t0
@@ -185,16 +232,143 @@ def cpu_baseline_all_cores(w, L, n_steps, procs=None):
                       % (procs, n_steps, loop, wall, cpu_model())}
 
 
-def gather_pattern_ceiling(roof):
+# ---------------------------------------------------------------------------
+# C2 headline and its relatives
+# ---------------------------------------------------------------------------
+
+def c2_headline(args, stream):
+    """The timed region: ``args.steps`` launches of ``args.launch_steps``
+    integration steps over the whole sweep after ``args.warmup`` untimed
+    launches; launch durations from HIP events on the plan's stream."""
+    import torch
+    from tvb_hip import _lib as L
+    B, dt, K = args.instances, 0.05, args.launch_steps
+    w, Lmat, d, H, G, sig = c2_setup(B)
+    nnz = int((w != 0).sum())
+    prec = {"mixed": L.MIXED, "f32": L.F32}[args.precision]
+    plan = build_c2_plan(w, d, G, sig, dt, K, prec, L.NOISE_PHILOX, 1234, tile=args.tile, waves=args.waves,
+                         stream=stream)
+    log("c2: N=68 nnz=%d H=%d B=%d, %d launches x %d steps, plan kernel=%d tile=%d waves=%d lds=%d"
+        % (nnz, H, B, args.steps, K, plan.kernel, plan.tile, plan.waves, plan.lds_bytes))
+    plan.init_uniform(99, [0.0, -2.0], [2.0, 1.5])
+    for k in range(args.warmup):
+        plan.run(K)
+        log("  warmup %d / %d, launch %.1f ms" % (k + 1, args.warmup, plan.last_run_ms()))
+    plan.sync()
+    torch.cuda.synchronize()
+    emitted = {"tavg_samples": 0, "bold_samples": 0}
+    t0 = time.perf_counter()
+    launches = []
+    for k in range(args.steps):
+        plan.run(K)
+        ms = plan.last_run_ms()                   # waits for this launch (HIP events on the plan's stream)
+        launches.append(ms)
+        emitted["tavg_samples"] += int(plan.layout(0)[3][0])
+        emitted["bold_samples"] += int(plan.layout(1)[3][0])
+        log("  %d / %d launches, %.1f ms" % (k + 1, args.steps, ms))
+    plan.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    flags = plan.flags()
+    nch = plan.nch
+    tile, waves, kernel = plan.tile, plan.waves, plan.kernel
+    plan.close()
+    rs = 68.0 * B * K * args.steps
+    avg_ms = float(np.mean(launches))
+    bpr = algorithmic_bytes_per_region_step(nnz, 68, nch, 4, 2, 20, 2, 80, 5000, 40000,
+                                            tavg_bytes=8 if args.precision == "mixed" else 4)
+    per_launch = bpr * 68 * B * K
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    tpr, tnote = measured_traffic(args.precision, nch)
+    dtype = ("mixed: f32 history ring, weights and coupling sums; f64 state, integration and monitors"
+             if args.precision == "mixed" else "f32 (state, history, coupling; f64 Bold sums)")
+    line = {
+        "metric": "region-steps/s batched FP32 vs plain FP64 at stated error; p50/p99 latency",
+        "value": rs / el, "unit": "region-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / max(args.steps, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": dtype,
+        "data": "synthetic (seeded connectome, parameter grid and device Philox noise; random initial history)",
+        "config": {"workload": "c2: MontbrioPazoRoxin HeunStochastic dt=0.05 ms sweep, 68 nodes x %d instances "
+                               "(128 G x 64 sigma), shared 25%%-density connectome (nnz %d, horizon %d), "
+                               "TemporalAverage 1 ms + Bold 2000 ms" % (B, nnz, H),
+                   "instances": B, "nodes": 68, "precision": args.precision, "kernel": kernel,
+                   "kernel_tile": tile, "kernel_waves": waves, "history_channels": nch},
+        "timed": {"what": "one step = one launch of %d integration steps (%.0f ms simulated) of all %d x 68 "
+                          "region states" % (K, K * dt, B),
+                  "integration_steps": K * args.steps, "simulated_ms": K * args.steps * dt,
+                  "region_steps": rs, "seconds": el, **emitted},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": tpr * 68 * B * K if tpr else None, "traffic_note": tnote,
+                     "kernel": "k_step (pipelined batch-lane loop)",
+                     "algorithmic_bytes_per_launch": per_launch, "algorithmic_bytes_per_region_step": bpr,
+                     "traffic_bytes_per_region_step": tpr, "avg_launch_ms": avg_ms,
+                     "median_launch_ms": float(np.median(launches)), "launches": len(launches)},
+        "nonfinite_instances": int((flags != 0).sum()),
+        "nonfinite_note": "instances whose state left the finite range (the high-G part of the grid diverges "
+                          "in the FP64 oracle as well); their work is done and counted like the others",
+    }
+    ctx = dict(w=w, Lmat=Lmat, d=d, H=H, G=G, sig=sig, nnz=nnz, bpr=bpr, avg_ms=avg_ms, value=rs / el, nch=nch)
+    return line, ctx
+
+
+def c2_rate(ctx, precision, launch_steps, n_launch=4, stream=None):
+    """Throughput of the full sweep in another precision: ``n_launch``
+    launches of ``launch_steps`` after one warm-up launch."""
+    from tvb_hip import _lib as L
+    plan = build_c2_plan(ctx["w"], ctx["d"], ctx["G"], ctx["sig"], 0.05, launch_steps, precision, L.NOISE_PHILOX,
+                         1234, stream=stream)
+    plan.init_uniform(99, [0.0, -2.0], [2.0, 1.5])
+    plan.run(launch_steps)
+    plan.sync()
+    t0 = time.perf_counter()
+    ms = []
+    for _ in range(n_launch):
+        plan.run(launch_steps)
+        ms.append(plan.last_run_ms())
+    plan.sync()
+    el = time.perf_counter() - t0
+    nch = plan.nch
+    plan.close()
+    B = len(ctx["G"])
+    bpr = algorithmic_bytes_per_region_step(ctx["nnz"], 68, nch, 4, 2, 20, 2, 80, 5000, 40000,
+                                            tavg_bytes=8 if precision == L.MIXED else 4)
+    avg = float(np.mean(ms))
+    return {"precision": PREC_NAMES[precision], "value": B * 68.0 * launch_steps * n_launch / el,
+            "unit": "region-steps/s", "avg_launch_ms": avg, "launches": n_launch, "launch_steps": launch_steps,
+            "roofline_frac": bpr * 68 * B * launch_steps / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
+def fp64_single_instance(ctx, G=1.0, sig=1e-3, n_steps=400000):
+    """Plain single-instance FP64 device path on the same connectome and
+    duration (20 s = 400 000 steps), the denominator of the batched ratio."""
+    from tvb_hip import _lib as L
+    plan = build_c2_plan(ctx["w"], ctx["d"], np.array([G]), np.array([sig]), 0.05, n_steps, L.F64,
+                         L.NOISE_PHILOX, 7)
+    plan.init_uniform(11, [0.0, -2.0], [2.0, 1.5])
+    plan.run(2000)
+    plan.sync()
+    t0 = time.perf_counter()
+    plan.run(n_steps)
+    plan.sync()
+    el = time.perf_counter() - t0
+    shape = [plan.kernel, plan.tile, plan.waves]
+    ntile, k = plan.tiles()
+    plan.close()
+    v = 68.0 * n_steps / el
+    return {"value": v, "unit": "region-steps/s", "steps": n_steps, "seconds": el, "kernel": shape,
+            "tiles": ntile, "split_block_steps": k, "batched_over_fp64_single": ctx["value"] / v}
+
+
+def gather_pattern_ceiling(line):
     """The device's rate for the step kernel's memory pattern (random
-    256-byte pieces, about 1/17 of the bytes written; tvbk_probe_gather)
-    and k_step's measured HBM rate (traffic per launch / launch time) as a
-    share of it; the same pattern with 1 KiB pieces for comparison."""
+    256-byte pieces, about 1/17 of the bytes written; tvbk_probe_gather)."""
     from tvb_hip import _lib as L
     r = {"unit": "GB/s",
          "reads_256B": L.probe_gather(256, False) / 1e9,
          "reads_256B_writes_1_17": L.probe_gather(256, True) / 1e9,
          "reads_1KiB_writes_1_17": L.probe_gather(1024, True) / 1e9}
+    roof = line["roofline"]
     if roof.get("traffic"):
         rate = roof["traffic"] / (roof["avg_launch_ms"] * 1e-3) / 1e9
         r["k_step_hbm_rate"] = rate
@@ -202,114 +376,161 @@ def gather_pattern_ceiling(roof):
     return r
 
 
-def c2_rate(w, d, G, sig, precision, n_steps=40000, chunk=20000, stream=None):
-    """Throughput of the full sweep in another precision: launches of
-    ``chunk`` steps after a warm-up, HIP-event launch times."""
-    from tvb_hip import _lib as L
-    plan = build_c2_plan(w, d, G, sig, 0.05, chunk, precision, L.NOISE_PHILOX, 1234, stream=stream)
-    plan.init_uniform(99, [0.0, -2.0], [2.0, 1.5])
-    plan.run(2000)
-    plan.sync()
-    t0 = time.perf_counter()
-    ms = []
-    for _ in range(max(1, n_steps // chunk)):
-        plan.run(chunk)
-        ms.append(plan.last_run_ms())
-    plan.sync()
-    el = time.perf_counter() - t0
-    plan.close()
-    return len(G) * w.shape[0] * chunk * len(ms) / el, float(np.mean(ms))
+# ---------------------------------------------------------------------------
+# error of the sweep against the FP64 oracle
+# ---------------------------------------------------------------------------
+
+# (G index, sigma index) of the checked instances on the 128 x 64 grid,
+# corners and interior, with the high-G / high-sigma corner
+ERROR_GRID = ((0, 0), (0, 63), (20, 40), (32, 10), (48, 50), (64, 0), (64, 63), (80, 25), (96, 63), (100, 5),
+              (115, 63), (118, 30), (121, 63), (124, 0), (127, 40), (127, 63))
 
 
-def fp64_single_instance(w, d, G, sig, n_steps):
-    """Plain single-instance FP64 device path on the same connectome."""
-    from tvb_hip import _lib as L
-    plan = build_c2_plan(w, d, np.array([G]), np.array([sig]), 0.05, n_steps, L.F64, L.NOISE_PHILOX, 7)
-    plan.init_uniform(11, [0.0, -2.0], [2.0, 1.5])
-    plan.run(min(2000, n_steps))
-    plan.sync()
-    t0 = time.perf_counter()
-    plan.run(n_steps)
-    plan.sync()
-    el = time.perf_counter() - t0
-    shape = (plan.kernel, plan.tile, plan.waves)
-    plan.close()
-    return w.shape[0] * n_steps / el, el, shape
+def _noise(seed, b, n_steps, N):
+    return np.random.default_rng([seed, b]).standard_normal((n_steps, 2, N))
 
 
-def error_sample(w, L, d, G, sig, precisions, n_steps=8000, B=32, check=(0, 17), bold=100.0):
-    """The batched kernel with injected noise vs the FP64 oracle on a sample
-    of the sweep (instances ``check`` of B spread over the grid),
-    TemporalAverage(1 ms) and Bold(``bold`` ms): max |err| / max |ref| per
-    monitor, in two scenarios of ``n_steps`` each:
-
-    * "random_ic": random initial history, as the sweep starts.  Its first
-      200 ms are a transient in which trajectories pass near the MPR saddle,
-      where any rounding difference grows by about 8% per step;
-      TemporalAverage is also reported after those 200 ms.
-    * "settled": initial history = the last horizon of states of the FP64
-      oracle's random_ic run, so the sample starts on settled dynamics."""
-    from tvb_hip import _lib as Lb
+def _oracle_error_child(inp, outp):
+    """Child process of the error sample: the FP64 oracle of one instance in
+    both scenarios (random initial history; then settled, starting from the
+    last horizon of the first run's states)."""
     from oracle.tvb_oracle import OracleSim
-    idx = np.linspace(0, len(G) - 1, B).astype(int)
-    Gs, ss = G[idx], sig[idx]
-    H = int(d.max()) + 1
+    a = np.load(inp)
+    w, L, ic = a["w"], a["L"], a["ic"]
+    G, s, n_steps, b, bold, H = float(a["G"]), float(a["sig"]), int(a["n_steps"]), int(a["b"]), float(a["bold"]), \
+        int(a["H"])
     N = w.shape[0]
+    res = {}
+
+    def run(init, z):
+        o = OracleSim(w, L, 5.0, 0.05, "MontbrioPazoRoxin", coupling_params=dict(a=G), integrator="HeunStochastic",
+                      nsig=np.array([s, s]), initial_conditions=init,
+                      monitors=(("TemporalAverage", 1.0, None), ("Bold", bold, None)), mode="fp64")
+        with np.errstate(all="ignore"):
+            return o.run(n_steps, noise=z, record_state=True)
+
+    ((_, t1), (_, b1)), st1 = run(ic, _noise(int(a["seed1"]), b, n_steps, N))
+    settled = st1[-H:]
+    ((_, t2), (_, b2)), st2 = run(settled, _noise(int(a["seed2"]), b, n_steps, N))
+    np.savez(outp, tavg1=t1[..., 0], bold1=b1[..., 0], tavg2=t2[..., 0], bold2=b2[..., 0], settled=settled,
+             finite1=np.isfinite(st1).all(), finite2=np.isfinite(st2).all())
+
+
+def error_sample(ctx, precisions, n_steps=42000, bold=2000.0, grid=ERROR_GRID, seeds=(7001, 7002)):
+    """The batched kernel (tile of 32, the headline's pipelined loop) with
+    injected noise vs the FP64 oracle on ``len(grid)`` instances of the
+    sweep, TemporalAverage(1 ms) and Bold(``bold`` ms, >= one emission):
+    max |err| / max |ref| per instance and monitor, max over the finite
+    instances, in two scenarios of ``n_steps`` each: "random_ic" (random
+    initial history, as the sweep starts: its first 200 ms are a transient
+    near the MPR saddle, also reported without them) and "settled" (initial
+    history = the last horizon of the FP64 oracle's random_ic states).
+    Instances whose oracle trajectory goes non-finite are listed with the
+    device's non-finite flag for them."""
+    import subprocess
+    import tempfile
+    from tvb_hip import _lib as Lb
+    w, Lmat, d, H, G, sig = ctx["w"], ctx["Lmat"], ctx["d"], ctx["H"], ctx["G"], ctx["sig"]
+    N = w.shape[0]
+    n_s = 64
+    idx = np.array([gi * n_s + si for gi, si in grid])
+    Gs, ss = G[idx], sig[idx]
+    B = len(idx)
     rng = np.random.default_rng(5)
     ic = np.empty((H, 2, N))
     ic[:, 0] = rng.uniform(0.0, 2.0, (H, N))
     ic[:, 1] = rng.uniform(-2.0, 1.5, (H, N))
-    z1 = rng.standard_normal((n_steps, 2, N, B))
-    z2 = rng.standard_normal((n_steps, 2, N, B))
+    t0 = time.perf_counter()
+    with tempfile.TemporaryDirectory() as tmp:
+        procs = []
+        for b in range(B):
+            inp, outp = os.path.join(tmp, "in%d.npz" % b), os.path.join(tmp, "out%d.npz" % b)
+            np.savez(inp, w=w, L=Lmat, ic=ic, G=Gs[b], sig=ss[b], n_steps=n_steps, b=b, bold=bold, H=H,
+                     seed1=seeds[0], seed2=seeds[1])
+            procs.append((subprocess.Popen([sys.executable, os.path.abspath(__file__), "--oracle-child", inp, outp],
+                                           stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, env=child_env(),
+                                           text=True), outp))
+            if len(procs) - sum(p.poll() is not None for p, _ in procs) >= host_procs():
+                procs[-host_procs()][0].wait()
+        # device runs of the random_ic scenario while the oracle children work
+        z1 = np.empty((n_steps, 2, N, B))
+        for b in range(B):
+            z1[..., b] = _noise(seeds[0], b, n_steps, N)
+        dev = {}
 
-    def oracle(b, init, z):
-        o = OracleSim(w, L, 5.0, 0.05, "MontbrioPazoRoxin", coupling_params=dict(a=float(Gs[b])),
-                      integrator="HeunStochastic", nsig=np.array([ss[b], ss[b]]), initial_conditions=init,
-                      monitors=(("TemporalAverage", 1.0, None), ("Bold", bold, None)), mode="fp64")
-        return o.run(n_steps, noise=z[..., b], record_state=True)
-
-    # per-instance initial histories of the settled scenario
-    refs = {"random_ic": {}, "settled": {}}
-    settled_ic = np.repeat(ic[..., np.newaxis], B, axis=-1)          # (H, 2, N, B)
-    for b in check:
-        res, states = oracle(b, ic, z1)
-        refs["random_ic"][b] = res
-        settled_ic[..., b] = states[-H:]
-        refs["settled"][b] = oracle(b, states[-H:], z2)[0]
-    out = {"instances": [int(i) for i in idx[list(check)]], "steps": n_steps, "bold_period_ms": bold}
-    for name in precisions:
-        prec = {"f32": Lb.F32, "mixed": Lb.MIXED}[name]
-        res = {}
-        for scen, z in (("random_ic", z1), ("settled", z2)):
-            plan = build_c2_plan(w, d, Gs, ss, 0.05, n_steps, prec, Lb.NOISE_INJECTED, 0, bold=bold, tile=32)
+        def device(prec, inits, z):
+            plan = build_c2_plan(w, d, Gs, ss, 0.05, n_steps, prec, Lb.NOISE_INJECTED, 0, bold=bold, tile=32,
+                                 kernel=Lb.KERNEL_BATCH_LANE)
             for b in range(B):
-                init = ic if scen == "random_ic" else settled_ic[..., b]
-                plan.set_history(b, init, init[-1], H - 1)
+                plan.set_history(b, inits[b], inits[b][-1], H - 1)
             plan.set_noise(z)
             plan.run(n_steps)
-            got_t, got_b = plan.read(0), plan.read(1)
+            out = (plan.read(0), plan.read(1), plan.flags(), [plan.kernel, plan.tile, plan.waves])
             plan.close()
-            et, et_late, eb = [], [], []
-            for b in check:
-                (tt, xt), (tb, xb) = refs[scen][b]
-                rt, rb = xt[..., 0], xb[..., 0]
-                dt_ = np.abs(got_t[b][1] - rt)
+            return out
+
+        precs = {"f32": Lb.F32, "mixed": Lb.MIXED}
+        for name in precisions:
+            dev[(name, "random_ic")] = device(precs[name], [ic] * B, z1)
+        del z1
+        refs = []
+        for p, outp in procs:
+            _, err = p.communicate()
+            if p.returncode != 0:
+                raise RuntimeError("oracle child failed: " + err[-400:])
+            refs.append(dict(np.load(outp)))
+        z2 = np.empty((n_steps, 2, N, B))
+        for b in range(B):
+            z2[..., b] = _noise(seeds[1], b, n_steps, N)
+        for name in precisions:
+            dev[(name, "settled")] = device(precs[name], [r["settled"] for r in refs], z2)
+        del z2
+    out = {"instances": [int(i) for i in idx], "G": [float(x) for x in Gs], "sigma": [float(x) for x in ss],
+           "steps": n_steps, "bold_period_ms": bold, "tavg_period_ms": 1.0, "error_bar": ERROR_BAR,
+           "oracle": "fp64 numpy restatement, one process per instance", "seconds": time.perf_counter() - t0}
+    for name in precisions:
+        res, ok = {}, True
+        for scen, key in (("random_ic", "1"), ("settled", "2")):
+            (tav, bol, flags, shape) = dev[(name, scen)]
+            et, et_late, eb, nonfinite = [], [], [], []
+            for b in range(B):
+                r = refs[b]
+                fin = bool(r["finite" + key]) and np.isfinite(r["tavg" + key]).all() and \
+                    np.isfinite(r["bold" + key]).all()
+                if not fin:
+                    nonfinite.append({"instance": int(idx[b]), "device_flag": int(flags[b]),
+                                      "device_tavg_finite": bool(np.isfinite(tav[b][1]).all())})
+                    continue
+                rt, rb = r["tavg" + key], r["bold" + key]
+                dt_ = np.abs(tav[b][1] - rt)
                 et.append(float(dt_.max() / np.abs(rt).max()))
                 et_late.append(float(dt_[200:].max() / np.abs(rt).max()))
-                eb.append(float(np.abs(got_b[b][1] - rb).max() / np.abs(rb).max()))
-            res[scen] = {"tavg_max_rel": max(et), "bold_max_rel": max(eb)}
+                eb.append(float(np.abs(bol[b][1] - rb).max() / np.abs(rb).max()))
+            res[scen] = {"tavg_max_rel": max(et) if et else None, "bold_max_rel": max(eb) if eb else None,
+                         "finite_instances": len(et), "nonfinite_in_oracle": nonfinite,
+                         "device_flags_agree": all(x["device_flag"] == 1 for x in nonfinite) and
+                         int(np.sum(flags != 0)) == len(nonfinite),
+                         "bold_samples": int(bol[0][1].shape[0]), "kernel": shape}
             if scen == "random_ic":
-                res[scen]["tavg_max_rel_after_200ms"] = max(et_late)
+                res[scen]["tavg_max_rel_after_200ms"] = max(et_late) if et_late else None
+            ok = ok and et and res[scen]["tavg_max_rel"] <= ERROR_BAR["tavg_max_rel"] and \
+                res[scen]["bold_max_rel"] <= ERROR_BAR["bold_max_rel"]
+        res["meets_error_bar"] = bool(ok)
         out[name] = res
     return out
 
 
-def c3_plan(n_inst=256, duration_scale=0.02, seed=303, stream=None):
+# ---------------------------------------------------------------------------
+# the other configurations
+# ---------------------------------------------------------------------------
+
+def c3_plan(n_inst=1024, duration_scale=1.0, seed=303, stream=None, max_steps=None, precision=None):
     """Uneven batch: JansenRit + SigmoidalJansenRit, EulerStochastic dt 0.1,
     node counts from {68, 96, 192, 400, 998}, a distinct seeded connectome
     per instance (density U[0.1, 0.3], lengths U[10, 240] mm at 3 mm/ms ->
     delays up to 800 steps), durations U[2, 30] s x duration_scale,
-    per-instance a, b ~ U[0.9, 1.1] x defaults, TemporalAverage(2 ms) + Bold."""
+    per-instance a, b ~ U[0.9, 1.1] x defaults, TemporalAverage(2 ms) + Bold.
+    Returns (plan, steps per instance, region-steps, info)."""
     from tvb_hip import _lib as L
     from tvb_hip.connectome import idelays, synthetic
     from tvb_hip.plan import CSR, MonitorSpec, Plan
@@ -329,29 +550,48 @@ def c3_plan(n_inst=256, duration_scale=0.02, seed=303, stream=None):
         p[3] *= rng.uniform(0.9, 1.1)
         params.append(p + [0.0, 0.005, 6.0, 0.56, 1.0] + [0.0, 0.0, 0.0, 1e-3, 1e-3, 1e-3])
     mons = [MonitorSpec(L.MON_TAVG, iround(2.0 / dt), [0, 1, 2, 3], name="TemporalAverage"),
-            make_bold_spec(dt, 2000.0)]
-    mons[1].voi = [0, 1, 2, 3]
-    plan = Plan(model=L.MODEL_JR, cfun=L.CFUN_SIGMOID_JR, scheme=L.EULER_STO, precision=L.F32,
+            make_bold_spec(dt, 2000.0, voi=(0, 1, 2, 3))]
+    plan = Plan(model=L.MODEL_JR, cfun=L.CFUN_SIGMOID_JR, scheme=L.EULER_STO,
+                precision=L.F32 if precision is None else precision,
                 noise=L.NOISE_PHILOX, conns=conns, params=np.array(params), dt=dt, obs_coef=np.eye(6)[:4],
-                monitors=mons, max_steps=max(steps), inst_conn=np.arange(n_inst, dtype=np.int32), seed=seed,
-                stream=stream)
+                monitors=mons, max_steps=max(steps) if max_steps is None else max_steps,
+                inst_conn=np.arange(n_inst, dtype=np.int32), seed=seed, stream=stream)
     plan.set_steps(np.array(steps, np.int64))
     lo = [-1.0, -500.0, -50.0, -6.0, -20.0, -500.0]
     plan.init_uniform(seed + 1, [x * 0.01 for x in lo], [-x * 0.01 for x in lo])
     rs = float(sum(c.n_node * s for c, s in zip(conns, steps)))
-    return plan, max(steps), rs, dict(instances=n_inst, duration_scale=duration_scale,
-                                      total_nnz=int(sum(c.nnz for c in conns)))
+    return plan, steps, rs, dict(instances=n_inst, duration_scale=duration_scale,
+                                 total_nnz=int(sum(c.nnz for c in conns)),
+                                 nodes_histogram={str(k): int(sum(c.n_node == k for c in conns))
+                                                  for k in (68, 96, 192, 400, 998)},
+                                 steps_min=int(min(steps)), steps_max=int(max(steps)))
 
 
-def run_c3(stream=None, n_inst=256, duration_scale=0.02):
-    plan, nmax, rs, info = c3_plan(n_inst, duration_scale, stream=stream)
+def run_c3(stream=None, n_inst=1024, duration_scale=1.0, chunk=20000):
+    """The uneven batch at its stated size: every instance runs its own
+    duration (2-30 s); the plan advances in calls of ``chunk`` steps (each
+    call's monitor samples stay on the device), instances stop at their
+    budgets.  Timed from the first call to the end of the last."""
     t0 = time.perf_counter()
-    plan.run(nmax)
+    plan, steps, rs, info = c3_plan(n_inst, duration_scale, stream=stream, max_steps=chunk)
+    build_s = time.perf_counter() - t0
+    ntile, k = plan.tiles()
+    plan.sync()
+    t0 = time.perf_counter()
+    ms = []
+    for _ in range((max(steps) + chunk - 1) // chunk):
+        plan.run(chunk)
+        ms.append(plan.last_run_ms())
+        log("  c3 call %d: %.1f ms" % (len(ms), ms[-1]))
     plan.sync()
     el = time.perf_counter() - t0
     flags = plan.flags()
+    shape = [plan.kernel, plan.tile, plan.waves]
+    tb = plan.time_block()
     plan.close()
-    info.update(value=rs / el, unit="region-steps/s", seconds=el, nonfinite=int((flags != 0).sum()))
+    info.update(value=rs / el, unit="region-steps/s", region_steps=rs, seconds=el, plan_build_s=build_s,
+                calls=len(ms), call_ms=[round(m, 1) for m in ms], tiles=ntile, split_block_steps=k,
+                time_block_steps=tb, kernel=shape, nonfinite=int((flags != 0).sum()))
     return info
 
 
@@ -442,7 +682,7 @@ def run_c2_zero_delay(B=8192, n_steps=40000, chunk=10000, stream=None, warmup=10
     flops = 2.0 * 68 * 2 * 68 * B * chunk
     tf = flops / (med * 1e-3) / 1e12
     out = {"value": 68.0 * B * n_steps / el, "unit": "region-steps/s", "steps": n_steps, "instances": B,
-           "kernel": [plan.kernel, plan.tile, plan.waves],
+           "precision": PREC_NAMES[prec], "kernel": [plan.kernel, plan.tile, plan.waves],
            "roofline": {"bound": "mfma", "achieved": tf, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                         "frac": tf / FP32_MFMA_PEAK_TFLOPS, "traffic": None,
                         "flops_per_region_step": 2.0 * 68 * 2, "median_launch_ms": med},
@@ -451,14 +691,13 @@ def run_c2_zero_delay(B=8192, n_steps=40000, chunk=10000, stream=None, warmup=10
     return out
 
 
-def run_balloon_on_c2(plan_chunk_steps=20000, B=8192):
-    """Balloon-Windkessel BOLD (FP64, on the device) of one C2 chunk's
-    TemporalAverage output: 1000 samples x 68 x 8192 series, timed with HIP
-    events around the kernel."""
+def run_balloon_on_c2(ctx, chunk_steps=20000, B=8192):
+    """Balloon-Windkessel BOLD (FP64, on the device) of one 20 000-step C2
+    launch's TemporalAverage output: 1000 samples x 68 x 8192 series, timed
+    with HIP events around the kernel, and its share of that launch."""
     import torch
-    from tvb_hip import _lib as L
     from tvb_hip.analyzers import BalloonModel, run_balloon
-    T, n = plan_chunk_steps // 20, 68 * B
+    T, n = chunk_steps // 20, 68 * B
     x = torch.rand((T, 2, n), dtype=torch.float64, device="cuda")
     y = torch.empty((T, n), dtype=torch.float64, device="cuda")
     st = torch.empty((4, n), dtype=torch.float64, device="cuda")
@@ -472,8 +711,10 @@ def run_balloon_on_c2(plan_chunk_steps=20000, B=8192):
     ev[1].record()
     torch.cuda.synchronize()
     ms = ev[0].elapsed_time(ev[1])
-    return {"samples": T, "series": n, "ms": ms, "series_samples_per_s": T * n / (ms * 1e-3),
-            "share_of_chunk": None}
+    out = {"samples": T, "series": n, "ms": ms, "series_samples_per_s": T * n / (ms * 1e-3)}
+    if ctx is not None and chunk_steps == 20000:
+        out["share_of_launch"] = ms / ctx["avg_ms"]
+    return out
 
 
 def run_copy(nbytes=2 << 30, reps=10):
@@ -496,168 +737,138 @@ def run_copy(nbytes=2 << 30, reps=10):
             "launches": reps + 1}
 
 
-def run_c4(n_requests=24, rate_hz=40.0, duration_scale=0.01, inst_range=(1, 64), seed=404):
+def run_c4(n_requests=16, rate_hz=None, duration_scale=1.0, inst_range=(1, 512), seed=404, est_rate=3e10,
+           busy=0.75, chunk_steps=20000):
     """Serving: Poisson stream of sweep requests (G2D or MPR, HeunStochastic,
-    N log-uniform in [68, 998], instances in inst_range, durations
-    U[5, 60] s x duration_scale), admitted by device footprint."""
+    N log-uniform in [68, 998], instances uniform in inst_range, durations
+    U[5, 60] s x duration_scale), admitted by device footprint.  Without
+    ``rate_hz`` the arrival rate is set so that the offered work keeps the
+    GPU about ``busy`` occupied at ``est_rate`` region-steps/s.  Each
+    chunk's monitor samples stay on the device (copy-out is not in the
+    latency)."""
     import torch
     from tvb_hip import _lib
     from tvb_hip.serve import Server, synthetic_requests
-    reqs = synthetic_requests(n_requests, rate_hz, seed=seed, inst_range=inst_range,
-                              duration_scale=duration_scale)
+    probe = synthetic_requests(n_requests, 1.0, seed=seed, inst_range=inst_range, duration_scale=duration_scale)
+    mean_rs = float(np.mean([r.region_steps for r in probe]))
+    if rate_hz is None:
+        rate_hz = busy * est_rate / mean_rs
+    reqs = synthetic_requests(n_requests, rate_hz, seed=seed, inst_range=inst_range, duration_scale=duration_scale)
     streams = [_lib.stream_create() for _ in range(4)]
     free, total = torch.cuda.mem_get_info()
-    srv = Server(hbm_budget=int(0.5 * free), streams=streams, chunk_steps=2000, keep_outputs=True)
+    srv = Server(hbm_budget=int(0.5 * free), streams=streams, chunk_steps=chunk_steps, keep_outputs=False, log=log)
     stats = srv.run(reqs)
     for st in streams:
         _lib.stream_destroy(st)
-    stats.update(duration_scale=duration_scale, rate_hz=rate_hz, inst_range=list(inst_range))
+    stats.update(duration_scale=duration_scale, rate_hz=rate_hz, inst_range=list(inst_range),
+                 chunk_steps=chunk_steps, arrival_rate_basis="%.2f x %.3g region-steps/s over a mean request of "
+                 "%.3g region-steps" % (busy, est_rate, mean_rs),
+                 requests=[dict(r.meta, rid=r.rid, latency_s=r.latency, region_steps=r.region_steps)
+                           for r in reqs])
     return stats
+
+
+# ---------------------------------------------------------------------------
+
+def guarded(name, fn, line):
+    log("extra %s ..." % name)
+    t0 = time.perf_counter()
+    try:
+        r = fn()
+    except Exception as exc:   # recorded in the line, the headline is kept
+        import traceback
+        log(traceback.format_exc())
+        r = {"error": "%s: %s" % (type(exc).__name__, exc)}
+    if isinstance(r, dict):
+        r.setdefault("extra_wall_s", time.perf_counter() - t0)
+    line[name] = r
+    log("extra %s done in %.1f s" % (name, time.perf_counter() - t0))
+    return r
+
+
+EXTRAS = ("other_precision", "fp64_single_instance", "error_vs_fp64_oracle", "gather_pattern_ceiling",
+          "single_instance_c1", "uneven_batch_c3", "serving_c4", "zero_delay_mfma", "balloon_bold_fp64",
+          "cpu_baseline", "cpu_baseline_all_cores")
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400000, help="timed integration steps (default: the full 20 s)")
-    ap.add_argument("--warmup", type=int, default=2000)
-    ap.add_argument("--chunk", type=int, default=20000, help="steps per kernel launch")
+    ap.add_argument("--steps", type=int, default=20, help="timed launches (default 20: the sweep's full 20 s)")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed launches before the timed ones")
+    ap.add_argument("--launch-steps", type=int, default=20000, help="integration steps per launch (one bench step)")
     ap.add_argument("--instances", type=int, default=8192)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=200000, help="oracle sample length (about 10-20 s of one host core)")
-    ap.add_argument("--no-extras", action="store_true", help="skip CPU baseline, FP64 single path and error sample")
+    ap.add_argument("--cpu-steps", type=int, default=200000, help="oracle sample length (about 10-20 s of one core)")
+    ap.add_argument("--no-extras", action="store_true", help="the headline only")
+    ap.add_argument("--extras", default=",".join(EXTRAS), help="comma-separated subset of: " + ",".join(EXTRAS))
     ap.add_argument("--workload", default="c2", choices=("c2", "c1", "c2z", "c3", "c4", "copy"))
-    ap.add_argument("--precision", default="f32", choices=("f32", "mixed"),
-                    help="f32: float32 state, history and coupling, float64 Bold (the sweep's precision); "
-                         "mixed: float32 history and coupling sums, float64 state (the reference's own mix)")
+    ap.add_argument("--precision", default="mixed", choices=("mixed", "f32"))
+    ap.add_argument("--oracle-child", nargs=2, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.oracle_child:
+        _oracle_error_child(*args.oracle_child)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU: every instance of the sweep is independent and the "
               "workload fits one MI355X; multi-GPU runs are not part of this build (--gpus %d)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.steps < 1 or args.launch_steps < 1:
+        sys.exit("--steps and --launch-steps must be positive")
 
     import torch
     from tvb_hip import _lib as L
     assert torch.cuda.is_available(), "no GPU"
     stream = torch.cuda.current_stream().cuda_stream
-    if args.workload == "c1":
-        print(json.dumps({"metric": "region-steps/s (single stochastic instance)", "workload": "c1",
-                          **run_c1()}), flush=True)
+    if args.workload != "c2":
+        fn = {"c1": lambda: run_c1(), "c3": lambda: run_c3(stream),
+              "c2z": lambda: run_c2_zero_delay(args.instances, stream=stream,
+                                               precision=L.MIXED if args.precision == "mixed" else L.F32),
+              "copy": lambda: run_copy(), "c4": lambda: run_c4()}[args.workload]
+        print(json.dumps({"workload": args.workload, **fn()}), flush=True)
         return
-    if args.workload == "c3":
-        print(json.dumps({"metric": "region-steps/s (uneven JansenRit batch)", "workload": "c3",
-                          **run_c3(stream)}), flush=True)
-        return
-    if args.workload == "c2z":
-        print(json.dumps({"metric": "region-steps/s (zero-delay sweep, dense MFMA)", "workload": "c2z",
-                          **run_c2_zero_delay(args.instances, stream=stream,
-                                              precision=L.MIXED if args.precision == "mixed" else L.F32)}),
-                         flush=True)
-        return
-    if args.workload == "copy":
-        print(json.dumps({"metric": "device copy bandwidth", "workload": "copy", **run_copy()}), flush=True)
-        return
-    if args.workload == "c4":
-        print(json.dumps({"metric": "p50/p99 request latency (serving)", "workload": "c4", **run_c4()}),
-              flush=True)
-        return
-    B = args.instances
-    dt = 0.05
-    w, Lmat, d, H, G, sig = c2_setup(B)
-    nnz = int((w != 0).sum())
-    log("workload c2: N=68 nnz=%d H=%d B=%d steps=%d" % (nnz, H, B, args.steps))
-    chunk = min(args.chunk, max(args.steps, 1))
+    line, ctx = c2_headline(args, stream)
+    log(json.dumps({k: line[k] for k in ("value", "ms_per_step", "roofline")}))
+    extras = [] if args.no_extras else [e for e in args.extras.split(",") if e]
     prec = L.MIXED if args.precision == "mixed" else L.F32
-    plan = build_c2_plan(w, d, G, sig, dt, max(chunk, args.warmup), prec, L.NOISE_PHILOX, 1234,
-                         tile=args.tile, waves=args.waves, stream=stream)
-    log("plan: kernel=%d tile=%d waves=%d lds=%d" % (plan.kernel, plan.tile, plan.waves, plan.lds_bytes))
-    plan.init_uniform(99, [0.0, -2.0], [2.0, 1.5])
-    # warmup
-    if args.warmup:
-        plan.run(args.warmup)
-    torch.cuda.synchronize()
-    plan.sync()
-    # timed region
-    t0 = time.perf_counter()
-    done, launches = 0, []
-    while done < args.steps:
-        n = min(chunk, args.steps - done)
-        plan.run(n)
-        ms = plan.last_run_ms()                   # waits for this launch (HIP events)
-        launches.append((n, ms))
-        done += n
-        log("  %d / %d steps, launch %.1f ms" % (done, args.steps, ms))
-    plan.sync()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    flags = plan.flags()
-    rs = 68.0 * B * args.steps
-    value = rs / el
-    # roofline of the step kernel
-    full = [(n, ms) for n, ms in launches if n == chunk] or launches
-    avg_ms = sum(ms for _, ms in full) / len(full)
-    med_ms = float(np.median([ms for _, ms in full]))
-    bpr = algorithmic_bytes_per_region_step(nnz, 68, 2, 4, 2, 20, 2, 80, 5000, 40000)
-    per_launch = bpr * 68 * B * full[0][0]
-    achieved = per_launch / (avg_ms * 1e-3) / 1e9
-    tpr = measured_traffic_per_region_step()
-    traffic = tpr * 68 * B * full[0][0] if tpr else None
-    line = {
-        "metric": "region-steps/s batched FP32 vs plain FP64 at stated error; p50/p99 latency",
-        "value": value, "unit": "region-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": el * 1e3 / max(args.steps, 1), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 history/coupling, f64 state" if args.precision == "mixed" else "f32",
-        "data": "synthetic",
-        "config": {"workload": "c2: MPR HeunStochastic dt=0.05 sweep, 68 nodes x %d instances, shared "
-                               "25%%-density connectome (nnz %d, horizon %d), TemporalAverage 1 ms + Bold 2000 ms, "
-                               "device Philox noise" % (B, nnz, H),
-                   "instances": B, "nodes": 68, "precision": args.precision, "kernel_tile": plan.tile,
-                   "kernel_waves": plan.waves,
-                   "steps_per_launch": chunk},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": per_launch, "algorithmic_bytes_per_region_step": bpr,
-                     "traffic_bytes_per_region_step": tpr, "avg_launch_ms": avg_ms, "median_launch_ms": med_ms,
-                     "launches": len(full)},
-        "nonfinite_instances": int((flags != 0).sum()),
+    other = L.F32 if prec == L.MIXED else L.MIXED
+    table = {
+        "other_precision": lambda: c2_rate(ctx, other, args.launch_steps, stream=stream),
+        "fp64_single_instance": lambda: fp64_single_instance(ctx),
+        "error_vs_fp64_oracle": lambda: error_sample(ctx, ("f32", "mixed")),
+        "gather_pattern_ceiling": lambda: gather_pattern_ceiling(line),
+        "single_instance_c1": lambda: run_c1(),
+        "uneven_batch_c3": lambda: run_c3(stream),
+        "serving_c4": lambda: run_c4(est_rate=ctx["value"]),
+        "zero_delay_mfma": lambda: run_c2_zero_delay(args.instances, stream=stream, precision=prec),
+        "balloon_bold_fp64": lambda: run_balloon_on_c2(ctx),
+        "cpu_baseline": lambda: cpu_baseline_dict(ctx, args.cpu_steps),
+        "cpu_baseline_all_cores": lambda: cpu_baseline_all_cores(ctx["w"], ctx["Lmat"], args.cpu_steps // 4),
     }
-    plan.close()
-    if not args.no_extras:
-        line["gather_pattern_ceiling"] = gather_pattern_ceiling(line["roofline"])
-        v64, el64, shape64 = fp64_single_instance(w, d, 1.0, 1e-3, 40000)
-        line["fp64_single_instance"] = {"value": v64, "unit": "region-steps/s", "steps": 40000,
-                                        "kernel": list(shape64)}
-        line["batched_fp32_over_fp64_single"] = value / v64
-        other = "mixed" if args.precision == "f32" else "f32"
-        v2, ms2 = c2_rate(w, d, G, sig, L.MIXED if other == "mixed" else L.F32, chunk=chunk, stream=stream)
-        line["other_precision"] = {"precision": other, "value": v2, "unit": "region-steps/s",
-                                   "avg_launch_ms": ms2, "roofline_frac": bpr * 68 * B * chunk / (ms2 * 1e-3) / 1e9
-                                   / HBM_PEAK_GBS, "steps": 40000}
-        errs = error_sample(w, Lmat, d, G, sig, ("f32", "mixed"))
-        line["error_vs_fp64_oracle"] = errs
-        e = errs[args.precision]
-        line["tavg_max_rel_err_vs_fp64_oracle"] = {
-            "settled": e["settled"]["tavg_max_rel"], "random_ic": e["random_ic"]["tavg_max_rel"],
-            "random_ic_after_200ms": e["random_ic"]["tavg_max_rel_after_200ms"]}
-        line["bold_max_rel_err_vs_fp64_oracle"] = {"settled": e["settled"]["bold_max_rel"],
-                                                   "random_ic": e["random_ic"]["bold_max_rel"]}
-        line["serving"] = run_c4()
-        line["p50_latency_s"] = line["serving"]["p50_latency_s"]
-        line["p99_latency_s"] = line["serving"]["p99_latency_s"]
-        line["uneven_batch"] = run_c3(stream)
-        line["single_instance_c1"] = run_c1()
-        line["zero_delay_mfma"] = run_c2_zero_delay(B, stream=stream, precision=prec)
-        bal = run_balloon_on_c2(chunk, B)
-        bal["share_of_chunk"] = bal["ms"] / avg_ms
-        line["balloon_bold_fp64"] = bal
-        cpu_rate, cpu_el = cpu_baseline(w, Lmat, 1.0, 1e-3, args.cpu_steps)
-        line["cpu_baseline"] = {"value": cpu_rate, "unit": "region-steps/s", "cores": 1, "kind": "port",
-                                "sample": "1 instance x %d steps (%.1f s simulated) of the c2 workload, "
-                                          "reference-faithful numpy oracle, 1 thread, %.1f s wall, %s"
-                                          % (args.cpu_steps, args.cpu_steps * dt / 1000.0, cpu_el, cpu_model())}
-        line["cpu_baseline_all_cores"] = cpu_baseline_all_cores(w, Lmat, args.cpu_steps // 4)
+    for name in extras:
+        if name not in table:
+            sys.exit("unknown extra %r" % name)
+        guarded(name, table[name], line)
+    # summary fields of the extras
+    f64 = line.get("fp64_single_instance", {})
+    if "value" in f64:
+        line["batched_over_fp64_single"] = f64["batched_over_fp64_single"]
+    err = line.get("error_vs_fp64_oracle", {})
+    if args.precision in err:
+        line["meets_error_bar"] = {p: err[p]["meets_error_bar"] for p in ("f32", "mixed") if p in err}
+    srv = line.get("serving_c4", {})
+    if "p50_latency_s" in srv:
+        line["p50_latency_s"], line["p99_latency_s"] = srv["p50_latency_s"], srv["p99_latency_s"]
     print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_dict(ctx, n_steps):
+    rate, el = cpu_baseline(ctx["w"], ctx["Lmat"], 1.0, 1e-3, n_steps)
+    return {"value": rate, "unit": "region-steps/s", "cores": 1, "kind": "port",
+            "sample": "1 instance x %d steps (%.1f s simulated) of the c2 workload, reference-faithful numpy "
+                      "oracle, 1 thread, %.1f s wall, %s" % (n_steps, n_steps * 0.05 / 1000.0, el, cpu_model())}
 
 
 if __name__ == "__main__":

@@ -314,6 +314,16 @@ int tvbk_read_flags(tvbk_plan *plan, int32_t *flags);
  * ((step - 1 - d + H) % H, history.py:252) into slots[nnz] (int32). */
 int tvbk_debug_slots(tvbk_plan *plan, int32_t conn, int64_t step, int32_t *slots);
 
+/* Debug: the standard normals the step kernels of a PHILOX plan draw for
+ * instance `inst` at steps step0 .. step0 + n_steps - 1 (its stream id,
+ * every node), z[n_steps][nsvar][n_node] float64 (host or device pointer):
+ * computed on the device with the kernels' generator in the plan's history
+ * precision (float32 for F32 / MIXED, widened exactly).  Injected into the
+ * oracle they reproduce the device's noise increments.  No reference
+ * counterpart (the reference draws from RandomState, noise.py:195-198). */
+int tvbk_debug_normals(tvbk_plan *plan, int32_t inst, int64_t step0, int64_t n_steps,
+                       double *z);
+
 /* Number of workgroups (tiles) of a launch and the block length of a plan
  * whose large edge-lane instances are split over several tiles by node
  * range (0: not split).  No reference counterpart (a property of this

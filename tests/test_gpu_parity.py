@@ -171,3 +171,26 @@ def test_mixed_rejects_sigmoidal_coupling():
                             nsig=[0.0, 0.0, 0.0, 0.01, 0.01, 0.01], spread=0.02)
     with pytest.raises(L.TvbkError):
         _backend().build_plan(sim, precision="mixed", max_steps=4)
+
+
+def test_jansen_rit_auto_precision_is_fp64():
+    """A drop-in run_sim of JansenRit + SigmoidalJansenRit with the default
+    precision ("auto") integrates in float64 (the reference's pre and sum
+    are float64, coupling.py:118-124, :370-373): max rel <= 1e-10 against
+    the pure-fp64 oracle on Raw, TemporalAverage and Bold."""
+    _say("jr auto precision")
+    mons = (("Raw", None, None), ("TemporalAverage", 1.0, None), ("Bold", 20.0, None))
+    nsig = [0.0, 0.0, 0.0, 0.01, 0.01, 0.01]
+    orc, sim, _ = make_case("JansenRit", n=20, density=0.3, lengths=(5.0, 60.0), speed=3.0, dt=0.1,
+                            scheme="EulerStochastic", coupling="SigmoidalJansenRit", nsig=nsig, monitors=mons,
+                            spread=0.02)
+    nstep = 1200
+    o = orc["fp64"]
+    z = np.random.default_rng(8).standard_normal((nstep, 6, o.N))
+    ref = o.run(nstep, noise=z)
+    out = _backend().run_sim(sim, nstep=nstep, noise=z)
+    for (tr, xr), (td, xd), name in zip(ref, out, ("raw", "tavg", "bold")):
+        assert xd.dtype == np.float64
+        e = rel_err(xd, xr)
+        _say("  ", name, "rel", e)
+        assert e <= 1e-10, (name, e)

@@ -194,3 +194,13 @@ def test_synthetic_connectome_properties():
     G, s = sweep_grid()
     assert G.shape == (8192,) and G.min() == 0.0 and G.max() == 2.0
     assert np.isclose(s.min(), 1e-4) and np.isclose(s.max(), 1e-2)
+
+
+def test_auto_precision_follows_the_reference_arithmetic():
+    """"auto" is the reference's mix for Linear coupling and float64 for
+    SigmoidalJansenRit (float32 only on request)."""
+    import tvb_hip._lib as L
+    from tvb_hip.backend import precision_id
+    assert precision_id("auto", L.CFUN_LINEAR) == L.MIXED
+    assert precision_id("auto", L.CFUN_SIGMOID_JR) == L.F64
+    assert precision_id("fp32", L.CFUN_SIGMOID_JR) == L.F32

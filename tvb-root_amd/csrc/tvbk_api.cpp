@@ -171,6 +171,7 @@ struct tvbk_plan {
   int64_t out_cap_samples[TVBK_MAX_MON] = {0};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool have_timing = false;
+  std::vector<uint64_t> stream_host;   // Philox stream id per instance
   std::vector<double> taps_host[TVBK_MAX_MON];
 };
 
@@ -834,6 +835,8 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   if (d.inst_stream) {
     if ((rc = up(&p->d_streams, d.inst_stream, (size_t)B * 8))) return bail(rc);
   }
+  p->stream_host.resize(B);
+  for (int b = 0; b < B; ++b) p->stream_host[b] = d.inst_stream ? d.inst_stream[b] : (uint64_t)b;
   std::vector<uint32_t> srec;   // host source of an async upload: alive until the final sync
   size_t srec_w_off = 0, srec_rp_off = 0;   // word offsets of the weights and row starts
   if (p->uniform_rows) {
@@ -1017,11 +1020,14 @@ int tvbk_set_history(tvbk_plan* p, int32_t inst, const void* history, int32_t el
     else reinterpret_cast<float*>(sbuf.data())[k] = (float)sin[k];
   }
   void *dh = nullptr, *ds = nullptr;
-  HIPCHK(p, hipMalloc(&dh, hbuf.size()));
-  HIPCHK(p, hipMalloc(&ds, sbuf.size()));
-  hipMemcpy(dh, hbuf.data(), hbuf.size(), hipMemcpyHostToDevice);
-  hipMemcpy(ds, sbuf.data(), sbuf.size(), hipMemcpyHostToDevice);
-  hipError_t e = hipSuccess;
+  hipError_t ea = hipMalloc(&dh, hbuf.size());
+  if (ea == hipSuccess) ea = hipMalloc(&ds, sbuf.size());
+  if (ea != hipSuccess) {
+    if (dh) hipFree(dh);
+    return fail(p, TVBK_ENOMEM, std::string("hipMalloc (history staging): ") + hipGetErrorString(ea));
+  }
+  hipError_t e = hipMemcpy(dh, hbuf.data(), hbuf.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(ds, sbuf.data(), sbuf.size(), hipMemcpyHostToDevice);
   for (int b = b0; b < b1 && e == hipSuccess; ++b) {
     const TileDesc& td = p->tiles[p->tile_of[b]];
     e = launch_scatter(p->d.precision, false, (char*)p->d_ring.p + td.ring_off * p->hsize,
@@ -1029,7 +1035,9 @@ int tvbk_set_history(tvbk_plan* p, int32_t inst, const void* history, int32_t el
                        p->lane_of[b], p->stream);
     p->cur_step[b] = current_step;
   }
-  hipStreamSynchronize(p->stream);
+  // the staging is consumed before it is freed or handed to the next call
+  const hipError_t es = hipStreamSynchronize(p->stream);
+  if (e == hipSuccess) e = es;
   hipFree(dh);
   hipFree(ds);
   if (e != hipSuccess) return fail(p, TVBK_EHIP, std::string("scatter: ") + hipGetErrorString(e));
@@ -1743,6 +1751,24 @@ int tvbk_read_flags(tvbk_plan* p, int32_t* flags) {
   if (!p || !flags) return fail(p, TVBK_EINVAL, "null argument");
   HIPCHK(p, hipStreamSynchronize(p->stream));
   HIPCHK(p, hipMemcpy(flags, p->d_flags.p, (size_t)p->d.n_inst * 4, hipMemcpyDefault));
+  return TVBK_OK;
+}
+
+int tvbk_debug_normals(tvbk_plan* p, int32_t inst, int64_t step0, int64_t n_steps, double* z) {
+  if (!p || !z) return fail(p, TVBK_EINVAL, "null argument");
+  if (p->d.noise != TVBK_NOISE_PHILOX) return fail(p, TVBK_ESTATE, "plan noise source is not PHILOX");
+  if (inst < 0 || inst >= p->d.n_inst) return fail(p, TVBK_EINVAL, "instance out of range");
+  if (n_steps <= 0 || step0 < 0) return fail(p, TVBK_EINVAL, "need n_steps > 0 and step0 >= 0");
+  const int N = p->conns[conn_of_inst(p, inst)].n_node, nsv = p->mi.nsv;
+  const size_t bytes = (size_t)n_steps * nsv * N * 8;
+  void* dz = nullptr;
+  HIPCHK(p, hipMalloc(&dz, bytes));
+  hipError_t e = launch_debug_normals(p->hsize == 8, nsv, N, p->stream_host[inst], step0, n_steps,
+                                      (uint32_t)p->d.seed, (uint32_t)(p->d.seed >> 32), (double*)dz, p->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+  if (e == hipSuccess) e = hipMemcpy(z, dz, bytes, hipMemcpyDefault);
+  hipFree(dz);
+  if (e != hipSuccess) return fail(p, TVBK_EHIP, std::string("debug normals: ") + hipGetErrorString(e));
   return TVBK_OK;
 }
 

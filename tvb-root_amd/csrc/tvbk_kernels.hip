@@ -193,9 +193,11 @@ hipError_t launch_areas(int esize, int mode, const void* raw, void* out, const A
 // [j][node][channel][T]) into ring slots (step0 + k0 + j) % H.
 template <class RH>
 __global__ void k_commit(const TileDesc* __restrict__ tiles, const ConnDesc* __restrict__ conns, RH* ring,
-                         const RH* __restrict__ stage, int T, int nch, int k0, int kn) {
-  const TileDesc td = tiles[blockIdx.y];
-  if (td.parts <= 1) return;
+                         const RH* __restrict__ stage, int T, int nch, int k0, int kn, int ntiles) {
+  // grid.y strides over the tiles (a plan may have more than 65535)
+  for (int ty = blockIdx.y; ty < ntiles; ty += gridDim.y) {
+  const TileDesc td = tiles[ty];
+  if (td.parts <= 1) continue;
   const int nb = (int)std::min<int64_t>(std::max<int64_t>(td.n_steps - k0, 0), kn);
   const ConnDesc cd = conns[td.conn];
   const int N = cd.n_node, H = cd.horizon;
@@ -210,6 +212,7 @@ __global__ void k_commit(const TileDesc* __restrict__ tiles, const ConnDesc* __r
     const int slot = (int)((td.step0 + k0 + j) % H);
     ring[td.ring_off + ring_index(T, H, N, nch, slot, i, ch, t)] = stage[td.stage_off + ((int64_t)j * N + i) * rs + e];
   }
+  }
 }
 
 hipError_t launch_commit(int hsize, const TileDesc* tiles, const ConnDesc* conns, void* ring, const void* stage,
@@ -218,10 +221,37 @@ hipError_t launch_commit(int hsize, const TileDesc* tiles, const ConnDesc* conns
   const dim3 grid(16, std::min(ntiles, 65535));
   if (hsize == 8)
     hipLaunchKernelGGL(k_commit<double>, grid, dim3(256), 0, st, tiles, conns, (double*)ring, (const double*)stage, T,
-                       nch, k0, kn);
+                       nch, k0, kn, ntiles);
   else
     hipLaunchKernelGGL(k_commit<float>, grid, dim3(256), 0, st, tiles, conns, (float*)ring, (const float*)stage, T,
-                       nch, k0, kn);
+                       nch, k0, kn, ntiles);
+  return hipGetLastError();
+}
+
+// Standard normals of the Philox noise source for steps step0 .. step0+n-1
+// of one instance stream, z[k][sv][node], with the step kernels' generator
+// (philox_normals in the history precision NR, widened exactly to double).
+template <class NR>
+__global__ void k_debug_normals(int nsv, int N, uint64_t stream, int64_t step0, int64_t n, uint32_t key0,
+                                uint32_t key1, double* z) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n * N; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = q / N;
+    const int i = (int)(q - k * N);
+    NR zz[8];
+    philox_normals<NR>(nsv, (uint32_t)i, stream, step0 + k, key0, key1, zz);
+    for (int v = 0; v < nsv; ++v) z[(k * nsv + v) * N + i] = (double)zz[v];
+  }
+}
+
+hipError_t launch_debug_normals(bool f64, int nsv, int N, uint64_t stream, int64_t step0, int64_t n, uint32_t key0,
+                                uint32_t key1, double* z, hipStream_t st) {
+  if (n <= 0 || N <= 0) return hipSuccess;
+  const int64_t work = n * N;
+  const int grid = (int)std::min<int64_t>((work + 255) / 256, 4096);
+  if (f64)
+    hipLaunchKernelGGL(k_debug_normals<double>, dim3(grid), dim3(256), 0, st, nsv, N, stream, step0, n, key0, key1, z);
+  else
+    hipLaunchKernelGGL(k_debug_normals<float>, dim3(grid), dim3(256), 0, st, nsv, N, stream, step0, n, key0, key1, z);
   return hipGetLastError();
 }
 

@@ -153,6 +153,9 @@ hipError_t launch_areas(int esize, int mode, const void* raw, void* out, const A
 // split tiles: staged history of a block's steps into the ring slots
 hipError_t launch_commit(int hsize, const TileDesc* tiles, const ConnDesc* conns, void* ring, const void* stage,
                          int T, int nch, int k0, int kn, int ntiles, hipStream_t st);
+// Philox normals of one instance stream, z[k][sv][node] (float64)
+hipError_t launch_debug_normals(bool f64, int nsv, int N, uint64_t stream, int64_t step0, int64_t n, uint32_t key0,
+                                uint32_t key1, double* z, hipStream_t st);
 hipError_t launch_debug_slots(const uint32_t* edges, int nnz, int H, int64_t step, int32_t* out, hipStream_t st);
 // Balloon-Windkessel BOLD, one thread per series
 hipError_t launch_balloon(const void* x, int x_elem_bytes, int64_t x_stride, int64_t n_samples, int64_t n_series,

@@ -93,6 +93,7 @@ def _bind(lib):
         "tvbk_import_monitor": ([P, C.c_int32, C.c_int32, P, P], C.c_int),
         "tvbk_read_flags": ([P, P], C.c_int),
         "tvbk_debug_slots": ([P, C.c_int32, C.c_int64, P], C.c_int),
+        "tvbk_debug_normals": ([P, C.c_int32, C.c_int64, C.c_int64, P], C.c_int),
         "tvbk_plan_shape": ([P, P, P, P, P], C.c_int),
         "tvbk_plan_tiles": ([P, P, P], C.c_int),
         "tvbk_plan_time_block": ([P, P], C.c_int),
@@ -116,7 +117,7 @@ EXPORTED = (
     "tvbk_set_steps", "tvbk_set_stimulus", "tvbk_sync", "tvbk_monitor_layout", "tvbk_read_monitor",
     "tvbk_monitor_device_ptr", "tvbk_export_state", "tvbk_export_monitor", "tvbk_import_monitor",
     "tvbk_read_flags",
-    "tvbk_debug_slots", "tvbk_plan_shape", "tvbk_plan_tiles", "tvbk_plan_time_block", "tvbk_last_run_ms",
+    "tvbk_debug_slots", "tvbk_debug_normals", "tvbk_plan_shape", "tvbk_plan_tiles", "tvbk_plan_time_block", "tvbk_last_run_ms",
     "tvbk_balloon",
     "tvbk_probe_gather", "tvbk_last_error")
 

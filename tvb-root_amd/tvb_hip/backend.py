@@ -36,11 +36,13 @@ PRECISIONS = {"fp32": L.F32, "float32": L.F32, "fp64": L.F64, "float64": L.F64, 
 
 
 def precision_id(precision, cfun_id):
-    """Plan precision for a name.  "auto" is the reference's own mix (float32
-    history and sums, float64 state: "mixed") where the device offers it,
-    float32 otherwise (SigmoidalJansenRit)."""
+    """Plan precision for a name.  "auto" is the reference's own arithmetic:
+    its mix (float32 history and sums, float64 state: "mixed") for Linear
+    coupling; float64 for SigmoidalJansenRit, whose reference pre and sum run
+    in float64 (coupling.py:118-124, :370-373) -- float32 stays an explicit
+    choice ("fp32")."""
     if precision == "auto":
-        return L.F32 if cfun_id == L.CFUN_SIGMOID_JR else L.MIXED
+        return L.F64 if cfun_id == L.CFUN_SIGMOID_JR else L.MIXED
     return PRECISIONS[precision]
 
 

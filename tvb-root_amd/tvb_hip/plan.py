@@ -183,8 +183,9 @@ class Plan:
         if history.dtype not in (np.float32, np.float64):
             history = history.astype(np.float64)
         state = np.ascontiguousarray(state, np.float64)
-        self._chk(self._lib.tvbk_set_history(self.handle, int(inst), L.ptr(history),
-                                             history.dtype.itemsize, L.ptr(state), int(current_step)))
+        fn = self._lib.tvbk_set_history
+        self._chk(fn(self.handle, int(inst), L.ptr(history), history.dtype.itemsize, L.ptr(state),
+                     int(current_step)))
 
     def init_uniform(self, seed, lo, hi):
         lo = np.ascontiguousarray(lo, np.float64)
@@ -345,6 +346,14 @@ class Plan:
         f = np.zeros(self.n_inst, np.int32)
         self._chk(self._lib.tvbk_read_flags(self.handle, L.ptr(f)))
         return f
+
+    def debug_normals(self, inst, step0, n_steps):
+        """Standard normals (n_steps, nsv, N) float64 the kernels of a Philox
+        plan draw for instance ``inst`` at steps step0 .. step0+n_steps-1."""
+        N = self._n_node(inst)
+        z = np.empty((int(n_steps), self.nsv, N), np.float64)
+        self._chk(self._lib.tvbk_debug_normals(self.handle, int(inst), int(step0), int(n_steps), L.ptr(z)))
+        return z
 
     def debug_slots(self, conn, step):
         out = np.zeros(max(self.conns[conn].nnz, 1), np.int32)

@@ -64,8 +64,9 @@ class Server:
     """Runs a list of requests with Poisson (or any) arrival times."""
 
     def __init__(self, hbm_budget, streams, chunk_steps=20000, max_active=16, clock=time.perf_counter,
-                 idle_sleep=2e-4, keep_outputs=True):
+                 idle_sleep=2e-4, keep_outputs=True, log=None):
         self.budget = int(hbm_budget)
+        self.log = log          # optional callable(str): a line per admission and retirement
         self.streams = list(streams)
         self.chunk = int(chunk_steps)
         self.max_active = int(max_active)
@@ -101,6 +102,8 @@ class Server:
                 si = min(stream_load, key=stream_load.get)
                 plan = r.build(self.streams[si], min(self.chunk, r.n_steps))
                 r.admitted = self.clock() - t0
+                if self.log:
+                    self.log("request %s admitted at %.2f s (arrived %.2f s)" % (r.rid, r.admitted, r.arrival))
                 a = _Active(r, plan, si, fp)
                 active.append(a)
                 stream_load[si] += 1
@@ -120,6 +123,8 @@ class Server:
                     r = a.req
                     r.flags = a.plan.flags()
                     r.finished = self.clock() - t0
+                    if self.log:
+                        self.log("request %s done at %.2f s, latency %.2f s" % (r.rid, r.finished, r.latency))
                     if self.keep_outputs:
                         r.outputs = _concat_chunks(a.chunks, a.plan.n_inst, len(a.plan.monitors))
                     a.plan.close()
@@ -235,4 +240,4 @@ def _make_request(rid, arrival, n, B, model, dur_ms, cseed, nseed, precision):
         return p
 
     return Request(rid, arrival, n_steps, build, footprint, region_steps=float(n) * B * n_steps,
-                   meta=dict(n=n, B=B, model=model, duration_ms=dur_ms, dt=dt))
+                   meta=dict(n=n, B=B, model=model, duration_ms=dur_ms, dt=dt, speed=speed, cseed=cseed))

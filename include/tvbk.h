@@ -219,6 +219,25 @@ int tvbk_set_history(tvbk_plan *plan, int32_t inst, const void *history,
                      int32_t elem_bytes, const double *state,
                      int64_t current_step);
 
+/* Allocation-free variant of tvbk_set_history: same arguments, effect and
+ * errors, but the device staging of the history and state goes through a
+ * static device array of TVBK_NOALLOC_MAX_BYTES bytes (one per device and
+ * process, held until the process ends) instead of per-call hipMalloc /
+ * hipFree; the call allocates no device memory.  It has no counterpart in
+ * the reference: it is for callers that cannot allocate -- a serving loop
+ * that re-initialises a running plan's instances between requests, code
+ * that must not call hipMalloc.  Limit: the history in the plan's ring
+ * precision, padded to 256 bytes, plus the state in its precision must fit
+ * TVBK_NOALLOC_MAX_BYTES, else TVBK_EINVAL.  Calls share the device's
+ * array: within a process they are serialised by a lock and each returns
+ * only after its staging has been consumed; calls from several processes
+ * each use their own process's array.  Every staged byte is written
+ * before it is read, so nothing needs clearing between calls. */
+#define TVBK_NOALLOC_MAX_BYTES (32 * 1024 * 1024)
+int tvbk_set_history_noalloc(tvbk_plan *plan, int32_t inst, const void *history,
+                             int32_t elem_bytes, const double *state,
+                             int64_t current_step);
+
 /* Fills every instance's history and state with uniform values in
  * [lo[sv], hi[sv]) from Philox (key seed, stream inst) on the device, bounded
  * like BaseHistory.from_simulator (history.py:150-152); current_step = 0.

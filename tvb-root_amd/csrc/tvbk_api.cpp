@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -20,6 +21,8 @@ using namespace tvbk;
 namespace {
 
 thread_local std::string g_err;
+// serialises the users of the static device scratch within this process
+std::mutex g_scratch_mu;
 
 struct ModelInfo {
   int nmp, nsv, ncv, ndp;
@@ -988,8 +991,12 @@ int tvbk_plan_tiles(tvbk_plan* p, int32_t* n_tiles, int32_t* split_k) {
 
 static int conn_of_inst(tvbk_plan* p, int b) { return p->tiles[p->tile_of[b]].conn; }
 
-int tvbk_set_history(tvbk_plan* p, int32_t inst, const void* history, int32_t elem_bytes, const double* state,
-                     int64_t current_step) {
+// History and state of instance(s) `inst` into the ring and tile images:
+// converted on the host to the plan's precisions, staged on the device, and
+// scattered into each instance's lane.  The staging is allocated per call,
+// or (noalloc) the static device scratch array of this process and device.
+static int set_history_impl(tvbk_plan* p, int32_t inst, const void* history, int32_t elem_bytes, const double* state,
+                            int64_t current_step, bool noalloc) {
   if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
   if (elem_bytes != 4 && elem_bytes != 8) return fail(p, TVBK_EINVAL, "elem_bytes must be 4 or 8");
   if (current_step < 0) return fail(p, TVBK_EINVAL, "current_step must be >= 0");
@@ -1020,11 +1027,25 @@ int tvbk_set_history(tvbk_plan* p, int32_t inst, const void* history, int32_t el
     else reinterpret_cast<float*>(sbuf.data())[k] = (float)sin[k];
   }
   void *dh = nullptr, *ds = nullptr;
-  hipError_t ea = hipMalloc(&dh, hbuf.size());
-  if (ea == hipSuccess) ea = hipMalloc(&ds, sbuf.size());
-  if (ea != hipSuccess) {
-    if (dh) hipFree(dh);
-    return fail(p, TVBK_ENOMEM, std::string("hipMalloc (history staging): ") + hipGetErrorString(ea));
+  std::unique_lock<std::mutex> scratch_lock;
+  if (noalloc) {
+    const size_t hb = (size_t)align_up((int64_t)hbuf.size(), 256);
+    if (hb + sbuf.size() > (size_t)TVBK_NOALLOC_MAX_BYTES)
+      return fail(p, TVBK_EINVAL, "history (" + std::to_string(hbuf.size()) + " bytes) and state (" +
+                                      std::to_string(sbuf.size()) + " bytes) exceed the static scratch of " +
+                                      std::to_string((long long)TVBK_NOALLOC_MAX_BYTES) + " bytes");
+    scratch_lock = std::unique_lock<std::mutex>(g_scratch_mu);
+    void* base = nullptr;
+    HIPCHK(p, scratch_device_ptr(&base));
+    dh = base;
+    ds = (unsigned char*)base + hb;
+  } else {
+    hipError_t ea = hipMalloc(&dh, hbuf.size());
+    if (ea == hipSuccess) ea = hipMalloc(&ds, sbuf.size());
+    if (ea != hipSuccess) {
+      if (dh) hipFree(dh);
+      return fail(p, TVBK_ENOMEM, std::string("hipMalloc (history staging): ") + hipGetErrorString(ea));
+    }
   }
   hipError_t e = hipMemcpy(dh, hbuf.data(), hbuf.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(ds, sbuf.data(), sbuf.size(), hipMemcpyHostToDevice);
@@ -1038,10 +1059,22 @@ int tvbk_set_history(tvbk_plan* p, int32_t inst, const void* history, int32_t el
   // the staging is consumed before it is freed or handed to the next call
   const hipError_t es = hipStreamSynchronize(p->stream);
   if (e == hipSuccess) e = es;
-  hipFree(dh);
-  hipFree(ds);
+  if (!noalloc) {
+    hipFree(dh);
+    hipFree(ds);
+  }
   if (e != hipSuccess) return fail(p, TVBK_EHIP, std::string("scatter: ") + hipGetErrorString(e));
   return TVBK_OK;
+}
+
+int tvbk_set_history(tvbk_plan* p, int32_t inst, const void* history, int32_t elem_bytes, const double* state,
+                     int64_t current_step) {
+  return set_history_impl(p, inst, history, elem_bytes, state, current_step, false);
+}
+
+int tvbk_set_history_noalloc(tvbk_plan* p, int32_t inst, const void* history, int32_t elem_bytes,
+                             const double* state, int64_t current_step) {
+  return set_history_impl(p, inst, history, elem_bytes, state, current_step, true);
 }
 
 int tvbk_export_state(tvbk_plan* p, int32_t inst, void* history, int32_t elem_bytes, double* state,

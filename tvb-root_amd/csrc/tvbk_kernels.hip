@@ -2,6 +2,7 @@
 // dispatch from the plan's runtime (precision, model, coupling) to the
 // instantiated step, init and dense kernels of tvbk_inst.hip.
 #include <algorithm>
+#include <mutex>
 
 #include "tvbk_kernels.hpp"
 #include "tvbk_device.hpp"
@@ -253,6 +254,31 @@ hipError_t launch_debug_normals(bool f64, int nsv, int N, uint64_t stream, int64
   else
     hipLaunchKernelGGL(k_debug_normals<float>, dim3(grid), dim3(256), 0, st, nsv, N, stream, step0, n, key0, key1, z);
   return hipGetLastError();
+}
+
+// Device memory of the allocation-free entry points: one array per device
+// (each GPU the library runs on holds its own copy until the process ends).
+// Host code never uses the array's name as an address: scratch_device_ptr
+// returns the device address of the current device's copy, looked up once
+// per device with hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[TVBK_SCRATCH_BYTES];
+
+hipError_t scratch_device_ptr(void** ptr) {
+  static std::mutex mu;
+  static void* addr[TVBK_SCRATCH_MAX_DEVICES] = {nullptr};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= TVBK_SCRATCH_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[dev] = p;
+  }
+  *ptr = addr[dev];
+  return hipSuccess;
 }
 
 hipError_t launch_debug_slots(const uint32_t* edges, int nnz, int H, int64_t step, int32_t* out, hipStream_t st) {

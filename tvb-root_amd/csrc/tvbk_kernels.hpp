@@ -3,9 +3,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "tvbk.h"
+
 #define TVBK_KMAX_MON 4
 #define TVBK_KMAX_OBS 8
 #define TVBK_KMAX_PAR 32
+// static device scratch of the allocation-free entry points (bytes), and the
+// devices per process it is looked up for
+#define TVBK_SCRATCH_BYTES TVBK_NOALLOC_MAX_BYTES
+#define TVBK_SCRATCH_MAX_DEVICES 64
 
 namespace tvbk {
 
@@ -153,6 +159,9 @@ hipError_t launch_areas(int esize, int mode, const void* raw, void* out, const A
 // split tiles: staged history of a block's steps into the ring slots
 hipError_t launch_commit(int hsize, const TileDesc* tiles, const ConnDesc* conns, void* ring, const void* stage,
                          int T, int nch, int k0, int kn, int ntiles, hipStream_t st);
+// Device address of the current device's static scratch array
+// (TVBK_SCRATCH_BYTES, for the allocation-free entry points).
+hipError_t scratch_device_ptr(void** ptr);
 // Philox normals of one instance stream, z[k][sv][node] (float64)
 hipError_t launch_debug_normals(bool f64, int nsv, int N, uint64_t stream, int64_t step0, int64_t n, uint32_t key0,
                                 uint32_t key1, double* z, hipStream_t st);

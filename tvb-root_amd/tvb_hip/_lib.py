@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libtvbk.so")
 ABI_VERSION = 2
 MAX_OBS = 8
 MAX_MON = 4
+NOALLOC_MAX_BYTES = 32 * 1024 * 1024   # TVBK_NOALLOC_MAX_BYTES
 
 # enums (mirror include/tvbk.h)
 OK, EINVAL, ENOMEM, EHIP, EUNSUPPORTED, ESTATE = 0, -1, -2, -3, -4, -5
@@ -79,6 +80,7 @@ def _bind(lib):
         "tvbk_stream_create": ([P], C.c_int),
         "tvbk_stream_destroy": ([P], C.c_int),
         "tvbk_set_history": ([P, C.c_int32, P, C.c_int32, P, C.c_int64], C.c_int),
+        "tvbk_set_history_noalloc": ([P, C.c_int32, P, C.c_int32, P, C.c_int64], C.c_int),
         "tvbk_init_uniform": ([P, C.c_uint64, P, P], C.c_int),
         "tvbk_set_noise": ([P, P, C.c_int64], C.c_int),
         "tvbk_run_steps": ([P, C.c_int64], C.c_int),
@@ -113,7 +115,7 @@ def _bind(lib):
 EXPORTED = (
     "tvbk_abi_version", "tvbk_model_info", "tvbk_plan_create", "tvbk_plan_destroy",
     "tvbk_plan_footprint", "tvbk_poll", "tvbk_stream_create", "tvbk_stream_destroy",
-    "tvbk_set_history", "tvbk_init_uniform", "tvbk_set_noise", "tvbk_run_steps",
+    "tvbk_set_history", "tvbk_set_history_noalloc", "tvbk_init_uniform", "tvbk_set_noise", "tvbk_run_steps",
     "tvbk_set_steps", "tvbk_set_stimulus", "tvbk_sync", "tvbk_monitor_layout", "tvbk_read_monitor",
     "tvbk_monitor_device_ptr", "tvbk_export_state", "tvbk_export_monitor", "tvbk_import_monitor",
     "tvbk_read_flags",

@@ -177,13 +177,15 @@ class Plan:
         except Exception:
             pass
 
-    def set_history(self, inst, history, state, current_step):
-        """history (H, nch, N) channel values; state (nsv, N) float64."""
+    def set_history(self, inst, history, state, current_step, noalloc=False):
+        """history (H, nch, N) channel values; state (nsv, N) float64.
+        ``noalloc``: stage through the library's static device scratch
+        (tvbk_set_history_noalloc, no device allocation)."""
         history = np.ascontiguousarray(history)
         if history.dtype not in (np.float32, np.float64):
             history = history.astype(np.float64)
         state = np.ascontiguousarray(state, np.float64)
-        fn = self._lib.tvbk_set_history
+        fn = self._lib.tvbk_set_history_noalloc if noalloc else self._lib.tvbk_set_history
         self._chk(fn(self.handle, int(inst), L.ptr(history), history.dtype.itemsize, L.ptr(state),
                      int(current_step)))
 

@@ -127,7 +127,10 @@ def make_bold_spec(dt, period=2000.0, voi=(0, 1)):
 
 
 def build_c2_plan(w, d, G, sig, dt, max_steps, precision, noise, seed, tile=0, waves=0, tavg=1.0, bold=2000.0,
-                  stream=None, kernel=0):
+                  stream=None, kernel=0, skip=False):
+    """The C2 sweep's plan; ``skip``: MontbrioPazoRoxin's V channel, whose
+    coupling coefficient cv is 0 in every instance, is not kept or gathered
+    (PLAN_SKIP_ZERO_COUPLING)."""
     from tvb_hip import _lib as L
     from tvb_hip.plan import CSR, MonitorSpec, Plan
     from tvb_hip.sim import iround
@@ -136,7 +139,8 @@ def build_c2_plan(w, d, G, sig, dt, max_steps, precision, noise, seed, tile=0, w
         mons.append(make_bold_spec(dt, bold))
     plan = Plan(model=L.MODEL_MPR, cfun=L.CFUN_LINEAR, scheme=L.HEUN_STO, precision=precision, noise=noise,
                 conns=[CSR(w, d)], params=mpr_params(G, sig), dt=dt, obs_coef=np.eye(2), monitors=mons,
-                max_steps=max_steps, seed=seed, tile=tile, waves=waves, stream=stream, kernel=kernel)
+                max_steps=max_steps, seed=seed, tile=tile, waves=waves, stream=stream, kernel=kernel,
+                flags=L.PLAN_SKIP_ZERO_COUPLING if skip else 0)
     return plan
 
 
@@ -246,8 +250,9 @@ def c2_headline(args, stream):
     w, Lmat, d, H, G, sig = c2_setup(B)
     nnz = int((w != 0).sum())
     prec = {"mixed": L.MIXED, "f32": L.F32}[args.precision]
+    skip = not args.all_channels
     plan = build_c2_plan(w, d, G, sig, dt, K, prec, L.NOISE_PHILOX, 1234, tile=args.tile, waves=args.waves,
-                         stream=stream)
+                         stream=stream, skip=skip)
     log("c2: N=68 nnz=%d H=%d B=%d, %d launches x %d steps, plan kernel=%d tile=%d waves=%d lds=%d"
         % (nnz, H, B, args.steps, K, plan.kernel, plan.tile, plan.waves, plan.lds_bytes))
     plan.init_uniform(99, [0.0, -2.0], [2.0, 1.5])
@@ -292,7 +297,9 @@ def c2_headline(args, stream):
                                "(128 G x 64 sigma), shared 25%%-density connectome (nnz %d, horizon %d), "
                                "TemporalAverage 1 ms + Bold 2000 ms" % (B, nnz, H),
                    "instances": B, "nodes": 68, "precision": args.precision, "kernel": kernel,
-                   "kernel_tile": tile, "kernel_waves": waves, "history_channels": nch},
+                   "kernel_tile": tile, "kernel_waves": waves, "history_channels": nch,
+                   "skipped_channels": ["V (MontbrioPazoRoxin cv = 0: its coupling term is 0 * c_V)"]
+                   if nch == 1 else []},
         "timed": {"what": "one step = one launch of %d integration steps (%.0f ms simulated) of all %d x 68 "
                           "region states" % (K, K * dt, B),
                   "integration_steps": K * args.steps, "simulated_ms": K * args.steps * dt,
@@ -308,16 +315,19 @@ def c2_headline(args, stream):
         "nonfinite_note": "instances whose state left the finite range (the high-G part of the grid diverges "
                           "in the FP64 oracle as well); their work is done and counted like the others",
     }
-    ctx = dict(w=w, Lmat=Lmat, d=d, H=H, G=G, sig=sig, nnz=nnz, bpr=bpr, avg_ms=avg_ms, value=rs / el, nch=nch)
+    ctx = dict(w=w, Lmat=Lmat, d=d, H=H, G=G, sig=sig, nnz=nnz, bpr=bpr, avg_ms=avg_ms, value=rs / el, nch=nch,
+               skip=skip)
     return line, ctx
 
 
-def c2_rate(ctx, precision, launch_steps, n_launch=4, stream=None):
-    """Throughput of the full sweep in another precision: ``n_launch``
-    launches of ``launch_steps`` after one warm-up launch."""
+def c2_rate(ctx, precision, launch_steps, n_launch=4, stream=None, skip=None):
+    """Throughput of the full sweep in another precision (or with all
+    history channels): ``n_launch`` launches of ``launch_steps`` after one
+    warm-up launch."""
     from tvb_hip import _lib as L
+    skip = ctx["skip"] if skip is None else skip
     plan = build_c2_plan(ctx["w"], ctx["d"], ctx["G"], ctx["sig"], 0.05, launch_steps, precision, L.NOISE_PHILOX,
-                         1234, stream=stream)
+                         1234, stream=stream, skip=skip)
     plan.init_uniform(99, [0.0, -2.0], [2.0, 1.5])
     plan.run(launch_steps)
     plan.sync()
@@ -334,7 +344,8 @@ def c2_rate(ctx, precision, launch_steps, n_launch=4, stream=None):
     bpr = algorithmic_bytes_per_region_step(ctx["nnz"], 68, nch, 4, 2, 20, 2, 80, 5000, 40000,
                                             tavg_bytes=8 if precision == L.MIXED else 4)
     avg = float(np.mean(ms))
-    return {"precision": PREC_NAMES[precision], "value": B * 68.0 * launch_steps * n_launch / el,
+    return {"precision": PREC_NAMES[precision], "history_channels": nch,
+            "value": B * 68.0 * launch_steps * n_launch / el,
             "unit": "region-steps/s", "avg_launch_ms": avg, "launches": n_launch, "launch_steps": launch_steps,
             "roofline_frac": bpr * 68 * B * launch_steps / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
@@ -344,7 +355,7 @@ def fp64_single_instance(ctx, G=1.0, sig=1e-3, n_steps=400000):
     duration (20 s = 400 000 steps), the denominator of the batched ratio."""
     from tvb_hip import _lib as L
     plan = build_c2_plan(ctx["w"], ctx["d"], np.array([G]), np.array([sig]), 0.05, n_steps, L.F64,
-                         L.NOISE_PHILOX, 7)
+                         L.NOISE_PHILOX, 7, skip=ctx["skip"])
     plan.init_uniform(11, [0.0, -2.0], [2.0, 1.5])
     plan.run(2000)
     plan.sync()
@@ -354,10 +365,12 @@ def fp64_single_instance(ctx, G=1.0, sig=1e-3, n_steps=400000):
     el = time.perf_counter() - t0
     shape = [plan.kernel, plan.tile, plan.waves]
     ntile, k = plan.tiles()
+    nch = plan.nch
     plan.close()
     v = 68.0 * n_steps / el
     return {"value": v, "unit": "region-steps/s", "steps": n_steps, "seconds": el, "kernel": shape,
-            "tiles": ntile, "split_block_steps": k, "batched_over_fp64_single": ctx["value"] / v}
+            "tiles": ntile, "split_block_steps": k, "history_channels": nch,
+            "batched_over_fp64_single": ctx["value"] / v}
 
 
 def gather_pattern_ceiling(line):
@@ -460,9 +473,9 @@ def error_sample(ctx, precisions, n_steps=42000, bold=2000.0, grid=ERROR_GRID, s
 
         def device(prec, inits, z):
             plan = build_c2_plan(w, d, Gs, ss, 0.05, n_steps, prec, Lb.NOISE_INJECTED, 0, bold=bold, tile=32,
-                                 kernel=Lb.KERNEL_BATCH_LANE)
+                                 kernel=Lb.KERNEL_BATCH_LANE, skip=ctx["skip"])
             for b in range(B):
-                plan.set_history(b, inits[b], inits[b][-1], H - 1)
+                plan.set_history(b, inits[b][:, :plan.nch], inits[b][-1], H - 1)
             plan.set_noise(z)
             plan.run(n_steps)
             out = (plan.read(0), plan.read(1), plan.flags(), [plan.kernel, plan.tile, plan.waves])
@@ -786,7 +799,7 @@ def guarded(name, fn, line):
     return r
 
 
-EXTRAS = ("other_precision", "fp64_single_instance", "error_vs_fp64_oracle", "gather_pattern_ceiling",
+EXTRAS = ("other_precision", "all_channels", "fp64_single_instance", "error_vs_fp64_oracle", "gather_pattern_ceiling",
           "single_instance_c1", "uneven_batch_c3", "serving_c4", "zero_delay_mfma", "balloon_bold_fp64",
           "cpu_baseline", "cpu_baseline_all_cores")
 
@@ -805,6 +818,8 @@ def main():
     ap.add_argument("--extras", default=",".join(EXTRAS), help="comma-separated subset of: " + ",".join(EXTRAS))
     ap.add_argument("--workload", default="c2", choices=("c2", "c1", "c2z", "c3", "c4", "copy"))
     ap.add_argument("--precision", default="mixed", choices=("mixed", "f32"))
+    ap.add_argument("--all-channels", action="store_true",
+                    help="keep MontbrioPazoRoxin's V channel (coupling coefficient cv = 0) in the history ring")
     ap.add_argument("--oracle-child", nargs=2, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.oracle_child:
@@ -836,6 +851,7 @@ def main():
     other = L.F32 if prec == L.MIXED else L.MIXED
     table = {
         "other_precision": lambda: c2_rate(ctx, other, args.launch_steps, stream=stream),
+        "all_channels": lambda: c2_rate(ctx, prec, args.launch_steps, stream=stream, skip=False),
         "fp64_single_instance": lambda: fp64_single_instance(ctx),
         "error_vs_fp64_oracle": lambda: error_sample(ctx, ("f32", "mixed")),
         "gather_pattern_ceiling": lambda: gather_pattern_ceiling(line),

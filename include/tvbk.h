@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define TVBK_ABI_VERSION 2
+#define TVBK_ABI_VERSION 3
 
 enum {
   TVBK_OK = 0,
@@ -172,7 +172,22 @@ typedef struct {
   const uint64_t *inst_stream;   /* [n_inst] PHILOX stream ids, NULL = index */
   int64_t max_steps;        /* largest n_steps of one tvbk_run_steps call */
   int32_t tile, waves;      /* kernel shape, 0 = auto */
+  uint32_t flags;           /* TVBK_PLAN_* */
 } tvbk_plan_desc;
+
+/* Plan flags.
+ *   SKIP_ZERO_COUPLING  coupling variables whose model coefficient is zero
+ *       in every instance and node are neither kept in the history ring nor
+ *       gathered: MontbrioPazoRoxin with Linear coupling and cv == 0 (its
+ *       V-coupling term is cv * c_V, infinite_theta.py:205) keeps only the r
+ *       channel, half the gather bytes.  The term the dfun reads becomes
+ *       0 * (a * 0 + b) instead of 0 * c_V: equal while c_V is finite.  The
+ *       plan's history is then its remaining channels only: tvbk_set_history
+ *       and tvbk_export_state take [H][n_channel][n_node] with the plan's
+ *       n_channel (tvbk_plan_channels).  Ignored (all channels kept) for
+ *       other models or couplings, the dense kernel and plans with afferent
+ *       monitors. */
+#define TVBK_PLAN_SKIP_ZERO_COUPLING 1u
 
 typedef struct tvbk_plan tvbk_plan;
 
@@ -354,6 +369,12 @@ int tvbk_plan_tiles(tvbk_plan *plan, int32_t *n_tiles, int32_t *split_k);
  * 0: one step at a time).  No reference counterpart (a property of this
  * backend's launch). */
 int tvbk_plan_time_block(tvbk_plan *plan, int32_t *steps);
+
+/* History channels of the plan (the model's coupling variables, one for
+ * SigmoidalJansenRit, fewer with TVBK_PLAN_SKIP_ZERO_COUPLING): the
+ * n_channel of tvbk_set_history / tvbk_export_state.  No reference
+ * counterpart (a property of this backend's plan). */
+int tvbk_plan_channels(tvbk_plan *plan, int32_t *n_channel);
 
 /* Kernel family, tile and waves chosen for the plan; LDS bytes per block. */
 int tvbk_plan_shape(tvbk_plan *plan, int32_t *kernel, int32_t *tile,

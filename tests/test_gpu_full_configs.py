@@ -38,22 +38,28 @@ def _say(*a):
 
 def _oracle_from_device(o, hist, state, current_step):
     """An oracle simulator continued from a device instance's exported
-    history ring (H, nch, N), state (nsv, N) and step."""
-    o.ring[...] = hist.astype(o.ring.dtype)
+    history ring (H, nch, N), state (nsv, N) and step.  Channels a plan does
+    not keep (zero coupling coefficient) are zero in the oracle's ring."""
+    o.ring[...] = 0
+    o.ring[:, :hist.shape[1]] = hist.astype(o.ring.dtype)
     o.current_state = np.asarray(state, np.float64)[..., np.newaxis].copy()
     o.current_step = int(current_step)
     return o
 
 
-def test_c2_full_shape_sampled_instances_bit_exact():
+@pytest.mark.parametrize("skip", [False, True], ids=["all-channels", "skip-zero-coupling"])
+def test_c2_full_shape_sampled_instances_bit_exact(skip):
+    """skip: the V channel (cv = 0 everywhere) is not kept; the oracle's V
+    history is then zero, which its zero coefficient makes irrelevant."""
     import bench
     from oracle.tvb_oracle import OracleSim
     from tvb_hip import _lib as L
     n_steps, B = 4000, 8192
     w, Lm, d, H, G, sig = bench.c2_setup(B)
-    plan = bench.build_c2_plan(w, d, G, sig, 0.05, n_steps, L.MIXED, L.NOISE_PHILOX, 1234, bold=100.0)
+    plan = bench.build_c2_plan(w, d, G, sig, 0.05, n_steps, L.MIXED, L.NOISE_PHILOX, 1234, bold=100.0, skip=skip)
     try:
         assert (plan.kernel, plan.tile) == (L.KERNEL_BATCH_LANE, 32)
+        assert plan.nch == (1 if skip else 2)
         plan.init_uniform(99, [0.0, -2.0], [2.0, 1.5])
         # candidates: spread over the grid plus the high-G corner (G >= 1.8)
         cand = sorted(set([0, 63, 1000, 2600, 4100, 4800, 6000, 7000] +
@@ -84,15 +90,51 @@ def test_c2_full_shape_sampled_instances_bit_exact():
                  "oracle finite", finite)
             assert bool(flags[b]) == (not finite), b
             np.testing.assert_array_equal(tav[k][0], tt)
-            np.testing.assert_array_equal(tav[k][1], xt[..., 0])
             np.testing.assert_array_equal(bol[k][0], tb)
+            if finite or not skip:
+                np.testing.assert_array_equal(tav[k][1], xt[..., 0])
+            else:
+                # past the divergence, 0 * c_V (NaN once V is) and the skip's
+                # 0 * b differ: equal up to the first non-finite sample
+                bad_d = ~np.isfinite(tav[k][1]).reshape(len(tt), -1).all(axis=1)
+                bad_o = ~np.isfinite(xt).reshape(len(tt), -1).all(axis=1)
+                assert bad_d.any() and bad_o.any()
+                j = min(np.argmax(bad_d), np.argmax(bad_o))
+                np.testing.assert_array_equal(tav[k][1][:j], xt[:j, ..., 0])
             if finite:
                 e = np.abs(bol[k][1] - xb[..., 0]).max() / np.abs(xb).max()
                 assert e <= 1e-12, (b, e)
-            else:
+            elif not skip:
                 np.testing.assert_array_equal(np.isfinite(bol[k][1]), np.isfinite(xb[..., 0]))
     finally:
         plan.close()
+
+
+def test_skip_zero_coupling_equals_all_channels_at_c2_shape():
+    """The V-channel skip changes nothing the reference computes: the full
+    8192-instance sweep with and without it, same initial draw and device
+    noise, gives bit-identical TemporalAverage and Bold on every instance
+    that stays finite, and the same non-finite flags."""
+    import bench
+    from tvb_hip import _lib as L
+    n_steps, B = 2000, 8192
+    w, Lm, d, H, G, sig = bench.c2_setup(B)
+    out = {}
+    for skip in (False, True):
+        plan = bench.build_c2_plan(w, d, G, sig, 0.05, n_steps, L.MIXED, L.NOISE_PHILOX, 1234, bold=50.0, skip=skip)
+        try:
+            plan.init_uniform(99, [0.0, -2.0], [2.0, 1.5])
+            plan.run(n_steps)
+            out[skip] = (plan.read_all(0)[0], plan.read_all(1)[0], plan.flags(), plan.nch)
+        finally:
+            plan.close()
+    (ta, ba, fa, na), (tb, bb, fb, nb) = out[False], out[True]
+    assert (na, nb) == (2, 1)
+    np.testing.assert_array_equal(fa, fb)
+    ok = fa == 0
+    _say("skip vs all channels: finite instances", int(ok.sum()), "of", B)
+    np.testing.assert_array_equal(ta.reshape(-1, B)[:, ok], tb.reshape(-1, B)[:, ok])
+    np.testing.assert_array_equal(ba.reshape(-1, B)[:, ok], bb.reshape(-1, B)[:, ok])
 
 
 def _dense48():

@@ -164,6 +164,47 @@ def test_mpr_dfun_closed_form():
     assert np.isclose(d[1], (V * V - (np.pi * tau * r) ** 2 + eta + J * tau * r + I + 0.8 * cr_ + 0.4 * cv_) / tau)
 
 
+def _mpr_fixed_points(tau, Delta, J, eta, I):
+    """Uncoupled MontbrioPazoRoxin fixed points from the model's equations
+    alone: dr = 0 gives V = -Delta / (2 pi tau r); dV = 0 then gives the
+    quartic -pi^2 tau^2 r^4 + J tau r^3 + (eta + I) r^2 + Delta^2 / (4 pi^2
+    tau^2) = 0, solved as a polynomial (numpy.roots) -- not by the oracle."""
+    roots = np.roots([-np.pi ** 2 * tau ** 2, J * tau, eta + I, 0.0, Delta ** 2 / (4 * np.pi ** 2 * tau ** 2)])
+    rs = sorted(r.real for r in roots if abs(r.imag) < 1e-9 and r.real > 0)
+    return [(r, -Delta / (2 * np.pi * tau * r)) for r in rs]
+
+
+@pytest.mark.parametrize("params", [{}, dict(tau=2.0, Delta=0.7, J=20.0, eta=-4.0, I=0.3)])
+def test_mpr_dfun_vanishes_at_independent_fixed_points(params):
+    """The oracle's MPR dfun (infinite_theta.py:171-207) is zero at the
+    fixed points the quartic fixes (two parameter sets), and, with the
+    default parameters, Model.stationary_trajectory (models/base.py:169-210,
+    as in models_test.py:174-243) converges to the two attracting ones from
+    nearby starts while the middle one (a saddle) is left.  A dropped term, a wrong sign or factor moves the fixed points
+    and fails both checks."""
+    from oracle.tvb_oracle import MODELS
+    p = dict(MODELS["MontbrioPazoRoxin"]["defaults"])
+    p.update(params)
+    fps = _mpr_fixed_points(p["tau"], p["Delta"], p["J"], p["eta"], p["I"])
+    assert len(fps) == 3
+    o = OracleSim(np.zeros((1, 1)), np.zeros((1, 1)), 1.0, 0.1, "MontbrioPazoRoxin", mode="fp64", model_params=params)
+    zero_c = np.zeros((2, 1, 1))
+    for r, V in fps:
+        d = o.dfun(np.array([[[r]], [[V]]]), zero_c)[:, 0, 0]
+        assert np.abs(d).max() < 1e-9 * max(1.0, abs(V) ** 2), (r, V, d)
+    if params:          # the convergence checks use the default parameters
+        return
+    lo, mid, hi = fps
+    for fp in (lo, hi):
+        for f in (0.95, 1.05):
+            y = stationary_trajectory("MontbrioPazoRoxin", [fp[0] * f, fp[1] * (2 - f)], coupling=np.zeros((2, 1)),
+                                      n_step=40000, dt=0.001, params=params)
+            np.testing.assert_allclose(y[-1].ravel(), fp, rtol=1e-4, atol=1e-5)
+    ends = [stationary_trajectory("MontbrioPazoRoxin", [mid[0] * f, mid[1] * (2 - f)], coupling=np.zeros((2, 1)),
+                                  n_step=40000, dt=0.001, params=params)[-1].ravel() for f in (0.9, 1.1)]
+    assert all(np.abs(e - np.array(mid)).max() > 0.1 for e in ends)
+
+
 def test_mpr_bound_applied():
     """r >= 0 after predictor and corrector (infinite_theta.py:129-134, integrators.py:355/360)."""
     ic = np.zeros((1, 2, 1))
@@ -317,6 +358,32 @@ def test_philox_known_answers(rounds, ctr, key, expected):
     """Random123 known-answer vectors for Philox4x32 (Salmon et al., SC'11)."""
     out = philox.philox4x32(np.array(ctr, np.uint32), np.array(key, np.uint32), rounds)
     assert [int(v) for v in out] == expected
+
+
+@pytest.mark.parametrize("precision,bits", [("f64", 32), ("f32", 24)])
+def test_box_muller_closed_form_cases(precision, bits):
+    """Box-Muller (Box and Muller 1958) on words that fix the answer:
+    u1 = (w0 + 1) 2^-b, u2 = w1 2^-b with b = 32 (f64) or the top 24 bits
+    (f32).  u1 = 1 gives radius 0; u2 = 0 gives (rad, 0), u2 = 1/4 gives
+    (0, rad), u2 = 1/2 gives (-rad, 0); the smallest u1 gives the largest
+    radius sqrt(2 b ln 2).  The same cases on the second pair (w2, w3); a
+    swapped u1/u2, sin/cos or pair order fails them."""
+    sh = 32 - bits
+    top = (1 << 32) - 1                        # w0 = 2^32 - 1: u1 = 1
+    quarter, half = 1 << 30, 1 << 31
+    words = np.array([[top, 0, 0, 0], [0, 0, top, quarter], [0, half, 0, quarter]], np.uint32)
+    z = philox.box_muller(words, precision)
+    rmax = np.sqrt(2.0 * bits * np.log(2.0))
+    np.testing.assert_allclose(z[0], [0.0, 0.0, rmax, 0.0], atol=1e-12)
+    np.testing.assert_allclose(z[1], [rmax, 0.0, 0.0, 0.0], atol=1e-12)
+    np.testing.assert_allclose(z[2], [-rmax, 0.0, 0.0, rmax], atol=1e-12)
+    # a generic word: the published formulas evaluated directly
+    w = np.array([[123456789, 987654321, 3141592653, 2718281828]], np.uint32)
+    u = (w.astype(np.float64) if sh == 0 else (w >> np.uint32(sh)).astype(np.float64)) * 2.0 ** -bits
+    r1, r2 = np.sqrt(-2 * np.log(u[0, 0] + 2.0 ** -bits)), np.sqrt(-2 * np.log(u[0, 2] + 2.0 ** -bits))
+    ref = [r1 * np.cos(2 * np.pi * u[0, 1]), r1 * np.sin(2 * np.pi * u[0, 1]),
+           r2 * np.cos(2 * np.pi * u[0, 3]), r2 * np.sin(2 * np.pi * u[0, 3])]
+    np.testing.assert_allclose(philox.box_muller(w, precision)[0], ref, rtol=1e-14)
 
 
 def test_philox_normals_statistics():

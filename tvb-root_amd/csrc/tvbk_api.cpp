@@ -115,6 +115,9 @@ struct tvbk_plan {
   tvbk_plan_desc d;
   ModelInfo mi;
   int nch = 0, nct = 0, ncp = 0, prow = 0;
+  // coupling function of the kernels: d.cfun, or CF_LINEAR1 when the plan
+  // skips the coupling variables whose coefficient is zero
+  int kcf = 0;
   int kernel = 0, T = 1, waves = 1, nthreads = 64;
   int esize = 4;    // state, parameters, Raw/TAvg outputs
   int hsize = 4;    // history ring, weights, coupling sums
@@ -357,8 +360,10 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
     p->err = "n_param " + std::to_string(d.n_param) + " != " + std::to_string(nparam_expect);
     return bail(TVBK_EINVAL);
   }
+  if (d.flags & ~(uint32_t)TVBK_PLAN_SKIP_ZERO_COUPLING) { p->err = "unknown plan flags"; return bail(TVBK_EINVAL); }
   p->nch = n_channel(d.model, d.cfun);
   p->nct = n_term(d.model, d.cfun);
+  p->kcf = d.cfun;
   p->prow = p->mi.ndp + p->ncp + p->mi.nsv;
   if (d.n_obs <= 0 || d.n_obs > TVBK_MAX_OBS || !d.obs_coef) { p->err = "bad observables"; return bail(TVBK_EINVAL); }
   if (d.n_mon < 0 || d.n_mon > TVBK_MAX_MON) { p->err = "bad monitor count"; return bail(TVBK_EINVAL); }
@@ -497,6 +502,23 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   }
   if (kernel == TVBK_KERNEL_BATCH_LANE && !shared) { p->err = "batch-lane kernel needs one shared connectome"; return bail(TVBK_EINVAL); }
   p->kernel = kernel;
+  // MontbrioPazoRoxin multiplies its V-coupling term by cv
+  // (infinite_theta.py:205): with cv == 0 in every instance (and node) the
+  // V channel is neither stored nor gathered, when the caller allows it
+  if ((d.flags & TVBK_PLAN_SKIP_ZERO_COUPLING) && d.model == TVBK_MODEL_MPR && d.cfun == TVBK_CFUN_LINEAR &&
+      kernel != TVBK_KERNEL_DENSE_MFMA && !any_aff) {
+    const int kcv = 7;   // raw MPR parameter cv
+    bool zero = true;
+    for (int b = 0; b < d.n_inst && zero; ++b) zero = d.params[(size_t)b * d.n_param + kcv] == 0.0;
+    if (zero && ((d.node_param_mask >> kcv) & 1ull) && d.node_params) {
+      const int N0 = d.conns[0].n_node;
+      for (int i = 0; i < N0 && zero; ++i) zero = d.node_params[(size_t)kcv * N0 + i] == 0.0;
+    }
+    if (zero) {
+      p->nch = p->nct = 1;
+      p->kcf = KCF_LINEAR1;
+    }
+  }
   int T = 1;
   if (kernel == TVBK_KERNEL_DENSE_MFMA) {
     if (d.tile && d.tile != 32) { p->err = "dense kernel uses tile 32"; return bail(TVBK_EINVAL); }
@@ -976,6 +998,12 @@ int tvbk_plan_shape(tvbk_plan* p, int32_t* kernel, int32_t* tile, int32_t* waves
   return TVBK_OK;
 }
 
+int tvbk_plan_channels(tvbk_plan* p, int32_t* n_channel) {
+  if (!p || !n_channel) return fail(p, TVBK_EINVAL, "null argument");
+  *n_channel = p->nch;
+  return TVBK_OK;
+}
+
 int tvbk_plan_time_block(tvbk_plan* p, int32_t* steps) {
   if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
   if (steps) *steps = p->tb_k;
@@ -1136,7 +1164,7 @@ int tvbk_init_uniform(tvbk_plan* p, uint64_t seed, const double* lo, const doubl
   for (int v = 0; v < p->mi.nsv; ++v) { a.lo[v] = lo[v]; a.hi[v] = hi[v]; }
   HIPCHK(p, hipMemcpyAsync(p->d_tiles.p, p->tiles.data(), p->tiles.size() * sizeof(TileDesc), hipMemcpyHostToDevice,
                            p->stream));
-  HIPCHK(p, launch_init(p->d.precision, p->d.model, p->d.cfun, a, (int)p->tiles.size(), p->stream));
+  HIPCHK(p, launch_init(p->d.precision, p->d.model, p->kcf, a, (int)p->tiles.size(), p->stream));
   HIPCHK(p, hipStreamSynchronize(p->stream));
   std::fill(p->cur_step.begin(), p->cur_step.end(), 0);
   return TVBK_OK;
@@ -1402,7 +1430,7 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
     for (int64_t k0 = 0; k0 < n; k0 += K) {
       a.k0 = (int32_t)k0;
       a.k1 = (int32_t)std::min<int64_t>(k0 + K, n);
-      HIPCHK(p, launch_step(d.precision, d.model, d.cfun, a, (int)p->tiles.size(), p->nthreads,
+      HIPCHK(p, launch_step(d.precision, d.model, p->kcf, a, (int)p->tiles.size(), p->nthreads,
                             (size_t)p->lds_total, p->stream));
       if (p->split_k > 0)
         HIPCHK(p, launch_commit(p->hsize, (const TileDesc*)p->d_tiles.p, (const ConnDesc*)p->d_conns.p,

@@ -14,7 +14,10 @@
 
 namespace tvbk {
 
-enum { CF_LINEAR = 0, CF_SIGMOID_JR = 1, CF_IDENTITY = 2 };
+// CF_LINEAR1: Linear coupling of the first coupling variable only -- the
+// kernel form of a plan whose model multiplies every other coupling term by
+// a zero coefficient (MontbrioPazoRoxin with cv = 0, infinite_theta.py:205)
+enum { CF_LINEAR = 0, CF_SIGMOID_JR = 1, CF_IDENTITY = 2, CF_LINEAR1 = 3 };
 enum { SCH_EULER_DET = 0, SCH_EULER_STO = 1, SCH_HEUN_DET = 2, SCH_HEUN_STO = 3, SCH_IDENTITY = 4 };
 enum { NOISE_NONE = 0, NOISE_INJECTED = 1, NOISE_PHILOX = 2 };
 enum { MON_RAW = 0, MON_TAVG = 1, MON_BOLD = 2 };
@@ -93,32 +96,42 @@ struct SUM {
 // Linear/IdCoupling, and the pre-synaptic sigmoid of (y1 - y2) for
 // SigmoidalJansenRit, whose pre depends on the source node only
 // (coupling.py:370-373), so it is applied once when the state is written.
-// NCH = history channels, NCT = coupling terms handed to the dfun.
+// NCH = history channels, NCT = coupling terms summed from them, NCD =
+// coupling terms the dfun reads (terms NCT .. NCD-1 are the post of a zero
+// sum).
 // derived coupling params: Linear a, b; SigmoidalJR cmin, cmax-cmin, r,
 // midpoint, a.
 // ---------------------------------------------------------------------------
 template <class M, int CF> struct Cfun;
 template <class M> struct Cfun<M, CF_LINEAR> {
-  static constexpr int NCH = M::NCV, NCT = M::NCV, NCP = 2;
+  static constexpr int NCH = M::NCV, NCT = M::NCV, NCD = M::NCV, NCP = 2;
   template <class R> __device__ static inline R channel(const R* x, int ch, const R* cp) { return x[M::cvar(ch)]; }
   template <class R> __device__ static inline R post(R gx, const R* cp) { return cp[0] * gx + cp[1]; }
   // parameters strided in LDS
   template <class R> __device__ static inline R post(R gx, const R* cp, int st) { return cp[0] * gx + cp[st]; }
 };
 template <class M> struct Cfun<M, CF_IDENTITY> {
-  static constexpr int NCH = M::NCV, NCT = M::NCV, NCP = 0;
+  static constexpr int NCH = M::NCV, NCT = M::NCV, NCD = M::NCV, NCP = 0;
   template <class R> __device__ static inline R channel(const R* x, int ch, const R* cp) { return x[M::cvar(ch)]; }
   template <class R> __device__ static inline R post(R gx, const R* cp) { return gx; }
   template <class R> __device__ static inline R post(R gx, const R* cp, int st) { return gx; }
 };
 template <class M> struct Cfun<M, CF_SIGMOID_JR> {
-  static constexpr int NCH = 1, NCT = 1, NCP = 5;
+  static constexpr int NCH = 1, NCT = 1, NCD = 1, NCP = 5;
   // SigmoidalJansenRit.pre (coupling.py:370-373) on x_j[:,0] - x_j[:,1]
   template <class R> __device__ static inline R channel(const R* x, int ch, const R* cp) {
     return cp[0] + cp[1] / (R(1) + exp(cp[2] * (cp[3] - (x[M::cvar(0)] - x[M::cvar(1)]))));
   }
   template <class R> __device__ static inline R post(R gx, const R* cp) { return cp[4] * gx; }
   template <class R> __device__ static inline R post(R gx, const R* cp, int st) { return cp[4 * st] * gx; }
+};
+// Linear on the first coupling variable: one history channel, one summed
+// term; the dfun's other terms are a * 0 + b (their coefficient is zero)
+template <class M> struct Cfun<M, CF_LINEAR1> {
+  static constexpr int NCH = 1, NCT = 1, NCD = M::NCV, NCP = 2;
+  template <class R> __device__ static inline R channel(const R* x, int ch, const R* cp) { return x[M::cvar(0)]; }
+  template <class R> __device__ static inline R post(R gx, const R* cp) { return cp[0] * gx + cp[1]; }
+  template <class R> __device__ static inline R post(R gx, const R* cp, int st) { return cp[0] * gx + cp[st]; }
 };
 
 // ---------------------------------------------------------------------------

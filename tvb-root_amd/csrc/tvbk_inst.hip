@@ -42,6 +42,9 @@ using InstR = float;
 #elif TVBK_INST_COMBO == 5
 #define TVBK_INST_M SUM
 #define TVBK_INST_CF CF_LINEAR
+#elif TVBK_INST_COMBO == 6
+#define TVBK_INST_M MPR
+#define TVBK_INST_CF CF_LINEAR1
 #else
 #error "unknown TVBK_INST_COMBO"
 #endif

@@ -62,6 +62,7 @@ static hipError_t dispatch(int model, int cf, bool init, const StepArgs* a, cons
   if (model == 2 && cf == CF_SIGMOID_JR) { TVBK_CASE(JR, CF_SIGMOID_JR) }
   if (model == 3 && cf == CF_IDENTITY) { TVBK_CASE(SUM, CF_IDENTITY) }
   if (model == 3 && cf == CF_LINEAR) { TVBK_CASE(SUM, CF_LINEAR) }
+  if (model == 1 && cf == CF_LINEAR1) { TVBK_CASE(MPR, CF_LINEAR1) }
 #undef TVBK_CASE
   return hipErrorInvalidValue;
 }

@@ -15,6 +15,10 @@
 
 namespace tvbk {
 
+// kernel coupling id of Linear on the first coupling variable only (plans
+// that skip zero-coefficient coupling channels; CF_LINEAR1 of the kernels)
+constexpr int KCF_LINEAR1 = 3;
+
 // One connectome as uploaded: CSR rows of packed (col | delay << 16) edges.
 struct ConnDesc {
   int32_t n_node, horizon, nnz, d_min, max_row, pad;

@@ -37,6 +37,8 @@
 
 namespace tvbk {
 
+static_assert(CF_LINEAR1 == KCF_LINEAR1, "the host's kernel coupling id");
+
 __device__ inline void copy_bytes(unsigned char* dst, const unsigned char* src, int64_t n) {
   // 16-byte granularity copy; all images are padded to 16 bytes.
   const int64_t n16 = n >> 4;
@@ -710,12 +712,14 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   // ring_out() stores them later.
   auto b_item = [&](int i, RH* cbuf, int k, int64_t s_, const StepInfo* sir, int wslot, bool stage,
                     const double* zpre) {
-    R c[NCT];
+    R c[C::NCD];
     R cpv[NCP > 0 ? NCP : 1];
 #pragma unroll
     for (int v = 0; v < NCP; ++v) cpv[v] = lane_param<R>(a, par, pst, NDP + v);
 #pragma unroll
     for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(i * NCT + v) * T + t], cpv);
+#pragma unroll
+    for (int v = NCT; v < C::NCD; ++v) c[v] = C::post(R(0), cpv);
     RH* wr = staged ? reinterpret_cast<RH*>(a.stage) + stage_off + ((int64_t)(k - kbeg) * N + i) * rstride + t
                     : ring + ring_index(T, H, N, NCH, wslot, i, 0, t);
     const int64_t wch = staged ? T : w_ch;
@@ -846,12 +850,14 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     __syncthreads();
     if (k == kfin) {
       for (int i = node0 + tid / T; i < node1; i += nthr / T) {
-        R c[NCT], x[NSV];
+        R c[C::NCD], x[NSV];
         R cpv[NCP > 0 ? NCP : 1];
 #pragma unroll
         for (int v = 0; v < NCP; ++v) cpv[v] = lane_param<R>(a, par, pst, NDP + v);
 #pragma unroll
         for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(i * NCT + v) * T + t], cpv);
+#pragma unroll
+        for (int v = NCT; v < C::NCD; ++v) c[v] = C::post(R(0), cpv);
 #pragma unroll
         for (int v = 0; v < NSV; ++v) x[v] = s_x[(i * NSV + v) * T + t];
         monitors_node<R, M, CF, true>(a, kc, smem, i, t, T, N, live, kfin, kc.si[buf], x, c, true);

@@ -32,7 +32,7 @@ FLAGS = [
 # and once for the dense kernels; every other source once.
 INST_SOURCE = "tvbk_inst.hip"
 INST_VARIANTS = ([("%s_c%d" % (("f32", "f64", "mixed")[r], c), ["-DTVBK_INST_R=%d" % r, "-DTVBK_INST_COMBO=%d" % c])
-                  for r in (0, 1, 2) for c in range(6)] + [("dense", ["-DTVBK_INST_DENSE"])])
+                  for r in (0, 1, 2) for c in range(7)] + [("dense", ["-DTVBK_INST_DENSE"])])
 
 
 def sources():

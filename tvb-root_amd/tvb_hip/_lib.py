@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtvbk.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAX_OBS = 8
 MAX_MON = 4
 NOALLOC_MAX_BYTES = 32 * 1024 * 1024   # TVBK_NOALLOC_MAX_BYTES
@@ -27,6 +27,7 @@ NOISE_NONE, NOISE_INJECTED, NOISE_PHILOX = 0, 1, 2
 MON_RAW, MON_TAVG, MON_BOLD = 0, 1, 2
 MON_SUBSAMPLE, MON_GAVG, MON_SAVG, MON_AFF, MON_AFF_TAVG = 3, 4, 5, 6, 7
 KERNEL_AUTO, KERNEL_BATCH_LANE, KERNEL_EDGE_LANE, KERNEL_DENSE_MFMA = 0, 1, 2, 3
+PLAN_SKIP_ZERO_COUPLING = 1
 
 
 class LibraryMissing(RuntimeError):
@@ -62,7 +63,7 @@ class PlanDesc(C.Structure):
                 ("node_param_mask", C.c_uint64), ("dt", C.c_double), ("n_obs", C.c_int32),
                 ("obs_coef", C.c_void_p), ("n_mon", C.c_int32), ("mon", Monitor * MAX_MON),
                 ("seed", C.c_uint64), ("inst_stream", C.c_void_p), ("max_steps", C.c_int64),
-                ("tile", C.c_int32), ("waves", C.c_int32)]
+                ("tile", C.c_int32), ("waves", C.c_int32), ("flags", C.c_uint32)]
 
 
 _lib = None
@@ -99,6 +100,7 @@ def _bind(lib):
         "tvbk_plan_shape": ([P, P, P, P, P], C.c_int),
         "tvbk_plan_tiles": ([P, P, P], C.c_int),
         "tvbk_plan_time_block": ([P, P], C.c_int),
+        "tvbk_plan_channels": ([P, P], C.c_int),
         "tvbk_last_run_ms": ([P, P], C.c_int),
         "tvbk_balloon": ([P, C.c_int32, C.c_int64, C.c_int64, C.c_int64, C.c_double, P, C.c_int32, C.c_int32,
                           P, P, P], C.c_int),
@@ -119,7 +121,7 @@ EXPORTED = (
     "tvbk_set_steps", "tvbk_set_stimulus", "tvbk_sync", "tvbk_monitor_layout", "tvbk_read_monitor",
     "tvbk_monitor_device_ptr", "tvbk_export_state", "tvbk_export_monitor", "tvbk_import_monitor",
     "tvbk_read_flags",
-    "tvbk_debug_slots", "tvbk_debug_normals", "tvbk_plan_shape", "tvbk_plan_tiles", "tvbk_plan_time_block", "tvbk_last_run_ms",
+    "tvbk_debug_slots", "tvbk_debug_normals", "tvbk_plan_shape", "tvbk_plan_tiles", "tvbk_plan_time_block", "tvbk_plan_channels", "tvbk_last_run_ms",
     "tvbk_balloon",
     "tvbk_probe_gather", "tvbk_last_error")
 

@@ -460,13 +460,18 @@ class HipBackend:
         return out
 
     def build_sweep_plan(self, sim, overrides, precision="auto", noise="philox", max_steps=65536,
-                         kernel="auto", tile=0, waves=0, seed=0, init="sim", init_seed=1, stream=None):
+                         kernel="auto", tile=0, waves=0, seed=0, init="sim", init_seed=1, stream=None,
+                         skip_zero_coupling=False):
         """A Plan of B instances of one simulator with per-instance parameter
         overrides, e.g. {"coupling.a": G, "noise.nsig": sigma, "model.eta": eta}
         (arrays of length B; "noise.nsig" sets every state variable).
         init="sim" starts every instance from the simulator's history;
         init="uniform" draws each instance's history on the device uniformly
-        in the model's state_variable_range (seeded by init_seed)."""
+        in the model's state_variable_range (seeded by init_seed).
+        skip_zero_coupling: coupling variables whose coefficient is zero in
+        every instance are not kept or gathered (MontbrioPazoRoxin with
+        cv = 0: the V channel, half the sweep's memory traffic); the plan's
+        history then holds the remaining channels (``plan.nch``)."""
         self.check_compatibility(sim)
         model_id, cfun_id, cfun_params, scheme, stochastic, specs = self._components(sim)
         prec = precision_id(precision, cfun_id)
@@ -496,7 +501,8 @@ class HipBackend:
         csr = CSR(sim.connectivity.weights, sim.connectivity.idelays, sim.connectivity.horizon)
         plan = Plan(model=model_id, cfun=cfun_id, scheme=scheme, precision=prec, noise=noise_id, conns=[csr],
                     params=params, dt=sim.integrator.dt, obs_coef=observables(sim.model), monitors=specs,
-                    max_steps=max_steps, seed=seed, kernel=kern, tile=tile, waves=waves, stream=stream)
+                    max_steps=max_steps, seed=seed, kernel=kern, tile=tile, waves=waves, stream=stream,
+                    flags=L.PLAN_SKIP_ZERO_COUPLING if skip_zero_coupling else 0)
         if init == "uniform":
             svr = sim.model.state_variable_range
             lo = [float(svr[sv][0]) for sv in sim.model.state_variables]
@@ -504,5 +510,5 @@ class HipBackend:
             plan.init_uniform(init_seed, lo, hi)
         else:
             hist, state = self._initial(sim, prec, cfun_id, csr)
-            plan.set_history(-1, hist, state, sim.current_step)
+            plan.set_history(-1, hist[:, :plan.nch], state, sim.current_step)
         return plan

@@ -83,7 +83,7 @@ class Plan:
     def __init__(self, *, model, cfun, scheme, precision, noise, conns, params, dt,
                  obs_coef, monitors, max_steps, inst_conn=None, node_params=None,
                  node_param_mask=0, seed=0, inst_stream=None, kernel=L.KERNEL_AUTO,
-                 tile=0, waves=0, stream=None, _dry=False):
+                 tile=0, waves=0, stream=None, flags=0, _dry=False):
         self._lib = L.load()
         self.handle = None
         self.model, self.cfun, self.scheme = model, cfun, scheme
@@ -135,6 +135,7 @@ class Plan:
         d.inst_stream = L.ptr(self.inst_stream)
         d.max_steps = self.max_steps
         d.tile, d.waves = tile, waves
+        d.flags = int(flags)
         self._desc = d
         self.esize = 8 if precision in (L.F64, L.MIXED) else 4   # state and Raw/TAvg outputs
         self.hsize = 8 if precision == L.F64 else 4                # history ring
@@ -150,6 +151,9 @@ class Plan:
         k, t, w, lds = (C.c_int32() for _ in range(4))
         L.check(self._lib.tvbk_plan_shape(h, C.byref(k), C.byref(t), C.byref(w), C.byref(lds)), h)
         self.kernel, self.tile, self.waves, self.lds_bytes = k.value, t.value, w.value, lds.value
+        nc = C.c_int32()
+        L.check(self._lib.tvbk_plan_channels(h, C.byref(nc)), h)
+        self.nch = nc.value          # history channels kept (fewer with PLAN_SKIP_ZERO_COUPLING)
 
     @classmethod
     def footprint(cls, **kw):

@@ -370,6 +370,15 @@ int tvbk_plan_tiles(tvbk_plan *plan, int32_t *n_tiles, int32_t *split_k);
  * backend's launch). */
 int tvbk_plan_time_block(tvbk_plan *plan, int32_t *steps);
 
+/* Calls of a split plan (tvbk_plan_tiles: split_k > 0) that run more than
+ * one block: a call repeating the previous call's length and kernel
+ * arguments is captured into a hipGraph of its block launches (step kernel
+ * and history commit per block), replayed while calls keep repeating them;
+ * TVBK_GRAPH=0 in the environment launches the blocks directly.
+ * Graphs captured and launched so far.  No reference counterpart (a
+ * property of this backend's launch). */
+int tvbk_plan_graphs(tvbk_plan *plan, int64_t *captured, int64_t *launched);
+
 /* History channels of the plan (the model's coupling variables, one for
  * SigmoidalJansenRit, fewer with TVBK_PLAN_SKIP_ZERO_COUPLING): the
  * n_channel of tvbk_set_history / tvbk_export_state.  No reference

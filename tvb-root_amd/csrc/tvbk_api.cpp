@@ -177,6 +177,17 @@ struct tvbk_plan {
   int64_t out_cap_samples[TVBK_MAX_MON] = {0};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool have_timing = false;
+  // hipGraph of a split call's block launches (step kernel and commit per
+  // block), captured on a private stream when a call repeats the length and
+  // kernel arguments of the call before it, and replayed while calls keep
+  // repeating them (all state the kernels read per call lives in device
+  // buffers that are updated before the replay)
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  StepArgs graph_args, prev_args;
+  int64_t graph_n = -1, prev_n = -1;
+  bool graph_off = false;                 // TVBK_GRAPH=0, or a capture failed
+  int64_t graph_captures = 0, graph_replays = 0;
   std::vector<uint64_t> stream_host;   // Philox stream id per instance
   std::vector<double> taps_host[TVBK_MAX_MON];
 };
@@ -265,6 +276,8 @@ static void destroy(tvbk_plan* p) {
   dev_free(&p->d_stim_tp);
   dev_free(&p->d_stage);
   dev_free(&p->d_dense);
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
   if (p->ev0) hipEventDestroy(p->ev0);
   if (p->ev1) hipEventDestroy(p->ev1);
   delete p;
@@ -949,6 +962,8 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   }
   HIPCHK(p, hipEventCreate(&p->ev0));
   HIPCHK(p, hipEventCreate(&p->ev1));
+  if (const char* env = std::getenv("TVBK_GRAPH"))
+    if (*env && std::atoi(env) == 0) p->graph_off = true;
   HIPCHK(p, hipStreamSynchronize(p->stream));
   *out = p;
   return TVBK_OK;
@@ -995,6 +1010,13 @@ int tvbk_plan_shape(tvbk_plan* p, int32_t* kernel, int32_t* tile, int32_t* waves
   if (tile) *tile = p->T;
   if (waves) *waves = p->waves;
   if (lds_bytes) *lds_bytes = p->lds_total;
+  return TVBK_OK;
+}
+
+int tvbk_plan_graphs(tvbk_plan* p, int64_t* captured, int64_t* launched) {
+  if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
+  if (captured) *captured = p->graph_captures;
+  if (launched) *launched = p->graph_replays;
   return TVBK_OK;
 }
 
@@ -1427,15 +1449,57 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
     // split plans run blocks of split_k steps, each followed by the commit
     // of the split tiles' staged history; others run the call in one launch
     const int64_t K = p->split_k > 0 ? p->split_k : std::max<int64_t>(n, 1);
-    for (int64_t k0 = 0; k0 < n; k0 += K) {
-      a.k0 = (int32_t)k0;
-      a.k1 = (int32_t)std::min<int64_t>(k0 + K, n);
-      HIPCHK(p, launch_step(d.precision, d.model, p->kcf, a, (int)p->tiles.size(), p->nthreads,
-                            (size_t)p->lds_total, p->stream));
-      if (p->split_k > 0)
-        HIPCHK(p, launch_commit(p->hsize, (const TileDesc*)p->d_tiles.p, (const ConnDesc*)p->d_conns.p,
-                                p->d_ring.p, p->d_stage.p, T, p->nch, (int)k0, (int)K, (int)p->tiles.size(),
-                                p->stream));
+    auto blocks = [&](hipStream_t st) -> hipError_t {
+      for (int64_t k0 = 0; k0 < n; k0 += K) {
+        StepArgs ab = a;
+        ab.k0 = (int32_t)k0;
+        ab.k1 = (int32_t)std::min<int64_t>(k0 + K, n);
+        hipError_t e = launch_step(d.precision, d.model, p->kcf, ab, (int)p->tiles.size(), p->nthreads,
+                                   (size_t)p->lds_total, st);
+        if (e == hipSuccess && p->split_k > 0)
+          e = launch_commit(p->hsize, (const TileDesc*)p->d_tiles.p, (const ConnDesc*)p->d_conns.p, p->d_ring.p,
+                            p->d_stage.p, T, p->nch, (int)k0, (int)K, (int)p->tiles.size(), st);
+        if (e != hipSuccess) return e;
+      }
+      return hipSuccess;
+    };
+    // several blocks: a call that repeats the previous call's length and
+    // arguments is captured into a graph, replayed while calls repeat it
+    bool graph = p->split_k > 0 && n > K && !p->graph_off;
+    const bool cached = graph && p->graph_exec && p->graph_n == n && std::memcmp(&a, &p->graph_args, sizeof(a)) == 0;
+    const bool repeat = graph && p->prev_n == n && std::memcmp(&a, &p->prev_args, sizeof(a)) == 0;
+    p->prev_args = a;
+    p->prev_n = n;
+    if (graph && !cached && !repeat) graph = false;
+    if (graph && !cached) {
+      if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+      p->graph_exec = nullptr;
+      hipError_t e = p->cap_stream ? hipSuccess : hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking);
+      hipGraph_t g = nullptr;
+      if (e == hipSuccess) e = hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        const hipError_t eb = blocks(p->cap_stream);
+        e = hipStreamEndCapture(p->cap_stream, &g);
+        if (eb != hipSuccess) e = eb;
+      }
+      if (e == hipSuccess) e = hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0);
+      if (g) (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {            // run the blocks directly from now on
+        p->graph_exec = nullptr;
+        p->graph_off = true;
+        graph = false;
+        (void)hipGetLastError();
+      } else {
+        p->graph_args = a;
+        p->graph_n = n;
+        ++p->graph_captures;
+      }
+    }
+    if (graph) {
+      HIPCHK(p, hipGraphLaunch(p->graph_exec, p->stream));
+      ++p->graph_replays;
+    } else {
+      HIPCHK(p, blocks(p->stream));
     }
   }
   // GlobalAverage / SpatialAverage of the call's node samples

@@ -251,13 +251,29 @@ __device__ __forceinline__ void monitors_node(const StepArgs& a, const KCfg& kc,
           if (mo.rat > 0) {
             // istep = rat * ii: the schedule of this push, from the step's
             // bookkeeping (consecutive slots, ages rat apart)
+            // in pairs: both accumulators are loaded before either is
+            // stored, two read-modify-writes in flight (the slots of one
+            // push are distinct; larger groups cost registers elsewhere)
+            constexpr int GRP = 2;
             int slot = si.fir_slot0, age = si.fir_age0;
-            for (int j = 0; j < si.fir_n; ++j) {
-              const double w = mo.taps[age == 0 ? mo.S - 1 : age - 1];
-              double* ap = acc + (((int64_t)slot * nv + v) * N + i) * T + t;
-              *ap = *ap + w * mean;
-              age += mo.rat;
-              slot = (slot + 1 == mo.Q) ? 0 : slot + 1;
+            const int64_t sstr = (int64_t)nv * N * T;           // doubles between slots
+            double* ap0 = acc + ((int64_t)v * N + i) * T + t;
+            for (int j0 = 0; j0 < si.fir_n; j0 += GRP) {
+              double* ap[GRP];
+              double av[GRP], wv[GRP];
+#pragma unroll
+              for (int q = 0; q < GRP; ++q) {
+                if (j0 + q < si.fir_n) {
+                  wv[q] = mo.taps[age == 0 ? mo.S - 1 : age - 1];
+                  ap[q] = ap0 + slot * sstr;
+                  av[q] = *ap[q];
+                  age += mo.rat;
+                  slot = (slot + 1 == mo.Q) ? 0 : slot + 1;
+                }
+              }
+#pragma unroll
+              for (int q = 0; q < GRP; ++q)
+                if (j0 + q < si.fir_n) *ap[q] = av[q] + wv[q] * mean;
             }
           } else {
             const int64_t klo = (mp * mo.ii + mo.istep - 1) / mo.istep;

@@ -101,6 +101,7 @@ def _bind(lib):
         "tvbk_plan_tiles": ([P, P, P], C.c_int),
         "tvbk_plan_time_block": ([P, P], C.c_int),
         "tvbk_plan_channels": ([P, P], C.c_int),
+        "tvbk_plan_graphs": ([P, P, P], C.c_int),
         "tvbk_last_run_ms": ([P, P], C.c_int),
         "tvbk_balloon": ([P, C.c_int32, C.c_int64, C.c_int64, C.c_int64, C.c_double, P, C.c_int32, C.c_int32,
                           P, P, P], C.c_int),
@@ -121,7 +122,7 @@ EXPORTED = (
     "tvbk_set_steps", "tvbk_set_stimulus", "tvbk_sync", "tvbk_monitor_layout", "tvbk_read_monitor",
     "tvbk_monitor_device_ptr", "tvbk_export_state", "tvbk_export_monitor", "tvbk_import_monitor",
     "tvbk_read_flags",
-    "tvbk_debug_slots", "tvbk_debug_normals", "tvbk_plan_shape", "tvbk_plan_tiles", "tvbk_plan_time_block", "tvbk_plan_channels", "tvbk_last_run_ms",
+    "tvbk_debug_slots", "tvbk_debug_normals", "tvbk_plan_shape", "tvbk_plan_tiles", "tvbk_plan_time_block", "tvbk_plan_channels", "tvbk_plan_graphs", "tvbk_last_run_ms",
     "tvbk_balloon",
     "tvbk_probe_gather", "tvbk_last_error")
 

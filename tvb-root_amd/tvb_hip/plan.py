@@ -342,6 +342,12 @@ class Plan:
         self._chk(self._lib.tvbk_plan_tiles(self.handle, C.byref(n), C.byref(k)))
         return n.value, k.value
 
+    def graphs(self):
+        """(hipGraphs captured, graph launches) of this plan's split calls."""
+        c, n = C.c_int64(), C.c_int64()
+        self._chk(self._lib.tvbk_plan_graphs(self.handle, C.byref(c), C.byref(n)))
+        return c.value, n.value
+
     def time_block(self):
         """Steps per block of the time-blocked sequential loop (0: off)."""
         k = C.c_int32()

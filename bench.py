@@ -127,7 +127,7 @@ def make_bold_spec(dt, period=2000.0, voi=(0, 1)):
 
 
 def build_c2_plan(w, d, G, sig, dt, max_steps, precision, noise, seed, tile=0, waves=0, tavg=1.0, bold=2000.0,
-                  stream=None, kernel=0, skip=False):
+                  stream=None, kernel=0, skip=False, tune=False):
     """The C2 sweep's plan; ``skip``: MontbrioPazoRoxin's V channel, whose
     coupling coefficient cv is 0 in every instance, is not kept or gathered
     (PLAN_SKIP_ZERO_COUPLING)."""
@@ -140,7 +140,7 @@ def build_c2_plan(w, d, G, sig, dt, max_steps, precision, noise, seed, tile=0, w
     plan = Plan(model=L.MODEL_MPR, cfun=L.CFUN_LINEAR, scheme=L.HEUN_STO, precision=precision, noise=noise,
                 conns=[CSR(w, d)], params=mpr_params(G, sig), dt=dt, obs_coef=np.eye(2), monitors=mons,
                 max_steps=max_steps, seed=seed, tile=tile, waves=waves, stream=stream, kernel=kernel,
-                flags=L.PLAN_SKIP_ZERO_COUPLING if skip else 0)
+                flags=L.PLAN_SKIP_ZERO_COUPLING if skip else 0, tune=tune)
     return plan
 
 
@@ -252,7 +252,8 @@ def c2_headline(args, stream):
     prec = {"mixed": L.MIXED, "f32": L.F32}[args.precision]
     skip = not args.all_channels
     plan = build_c2_plan(w, d, G, sig, dt, K, prec, L.NOISE_PHILOX, 1234, tile=args.tile, waves=args.waves,
-                         stream=stream, skip=skip)
+                         stream=stream, skip=skip, tune=not args.no_tune)
+    tuned = plan.tuned
     log("c2: N=68 nnz=%d H=%d B=%d, %d launches x %d steps, plan kernel=%d tile=%d waves=%d lds=%d"
         % (nnz, H, B, args.steps, K, plan.kernel, plan.tile, plan.waves, plan.lds_bytes))
     plan.init_uniform(99, [0.0, -2.0], [2.0, 1.5])
@@ -298,6 +299,8 @@ def c2_headline(args, stream):
                                "TemporalAverage 1 ms + Bold 2000 ms" % (B, nnz, H),
                    "instances": B, "nodes": 68, "precision": args.precision, "kernel": kernel,
                    "kernel_tile": tile, "kernel_waves": waves, "history_channels": nch,
+                   "kernel_shape_from": "autotuner (trial launches of 2000 steps)" if tuned else "built-in heuristic",
+                   "autotune": tuned,
                    "skipped_channels": ["V (MontbrioPazoRoxin cv = 0: its coupling term is 0 * c_V)"]
                    if nch == 1 else []},
         "timed": {"what": "one step = one launch of %d integration steps (%.0f ms simulated) of all %d x 68 "
@@ -818,6 +821,7 @@ def main():
     ap.add_argument("--extras", default=",".join(EXTRAS), help="comma-separated subset of: " + ",".join(EXTRAS))
     ap.add_argument("--workload", default="c2", choices=("c2", "c1", "c2z", "c3", "c4", "copy"))
     ap.add_argument("--precision", default="mixed", choices=("mixed", "f32"))
+    ap.add_argument("--no-tune", action="store_true", help="the built-in kernel shape instead of the autotuner's")
     ap.add_argument("--all-channels", action="store_true",
                     help="keep MontbrioPazoRoxin's V channel (coupling coefficient cv = 0) in the history ring")
     ap.add_argument("--oracle-child", nargs=2, help=argparse.SUPPRESS)

@@ -39,7 +39,16 @@ struct ConnDesc {
 // channels x instances); single-instance tiles (T = 1) keep a node's slots
 // together ([node][channel][slot]: consecutive steps of one edge read
 // consecutive values, so a time block's gathers of an edge share lines).
+// Host builds with TVBK_BOUNDS_CHECK assert that every coordinate lies in
+// its range (the CPU check of the plans' addressing).
+#if defined(TVBK_BOUNDS_CHECK) && !defined(__HIP_DEVICE_COMPILE__)
+#define TVBK_CHECK(cond) ((cond) ? (void)0 : bounds_failed(#cond, __FILE__, __LINE__))
+void bounds_failed(const char* cond, const char* file, int line);
+#else
+#define TVBK_CHECK(cond) ((void)0)
+#endif
 __host__ __device__ inline int64_t ring_index(int T, int H, int N, int nch, int slot, int i, int ch, int t) {
+  TVBK_CHECK(slot >= 0 && slot < H && i >= 0 && i < N && ch >= 0 && ch < nch && t >= 0 && t < T);
   return T == 1 ? ((int64_t)i * nch + ch) * H + slot : (((int64_t)slot * N + i) * nch + ch) * T + t;
 }
 

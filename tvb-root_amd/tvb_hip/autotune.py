@@ -23,7 +23,17 @@ def signature(kw, device=""):
     mons = [(m.kind, m.istep, len(m.voi)) for m in kw["monitors"]]
     return json.dumps(dict(dev=device, model=kw["model"], cfun=kw["cfun"], scheme=kw["scheme"],
                            prec=kw["precision"], noise=kw["noise"], n=c.n_node, nnz=c.nnz, h=c.horizon,
-                           b=int(np.asarray(kw["params"]).shape[0]), mons=mons), sort_keys=True)
+                           b=int(np.asarray(kw["params"]).shape[0]), mons=mons, flags=int(kw.get("flags", 0))),
+                      sort_keys=True)
+
+
+def device_name():
+    """Name of the current GPU (part of the cache key)."""
+    try:
+        import torch
+        return torch.cuda.get_device_name(0)
+    except Exception:
+        return ""
 
 
 def load_cache(path=CACHE_PATH):
@@ -44,7 +54,7 @@ def save_cache(cache, path=CACHE_PATH):
     os.replace(tmp, path)
 
 
-def tune(plan_kwargs, trial_steps=2000, candidates=DEFAULT_CANDIDATES, device="", cache_path=CACHE_PATH,
+def tune(plan_kwargs, trial_steps=2000, candidates=DEFAULT_CANDIDATES, device="", cache_path=None,
          make_plan=None, init=None, use_cache=True):
     """Best (tile, waves) for the batch-lane kernel of this workload.
 
@@ -54,6 +64,8 @@ def tune(plan_kwargs, trial_steps=2000, candidates=DEFAULT_CANDIDATES, device=""
     Returns (tile, waves, table) with table = {(tile, waves): ms per step}."""
     from . import _lib as L
     from .plan import Plan
+    plan_kwargs = {k: v for k, v in plan_kwargs.items() if k not in ("tile", "waves", "max_steps", "kernel", "tune")}
+    cache_path = CACHE_PATH if cache_path is None else cache_path
     key = signature(plan_kwargs, device)
     cache = load_cache(cache_path) if use_cache else {}
     if use_cache and key in cache:
@@ -82,5 +94,8 @@ def tune(plan_kwargs, trial_steps=2000, candidates=DEFAULT_CANDIDATES, device=""
     best = min(table, key=table.get)
     if use_cache:
         cache[key] = {"best": list(best), "table": [[list(k), v] for k, v in table.items()]}
-        save_cache(cache, cache_path)
+        try:
+            save_cache(cache, cache_path)
+        except OSError:          # a read-only home: the choice is still returned
+            pass
     return best[0], best[1], table

@@ -83,8 +83,25 @@ class Plan:
     def __init__(self, *, model, cfun, scheme, precision, noise, conns, params, dt,
                  obs_coef, monitors, max_steps, inst_conn=None, node_params=None,
                  node_param_mask=0, seed=0, inst_stream=None, kernel=L.KERNEL_AUTO,
-                 tile=0, waves=0, stream=None, flags=0, _dry=False):
+                 tile=0, waves=0, stream=None, flags=0, tune=False, _dry=False):
+        """tune: pick the batch-lane kernel's (tile, waves) with the
+        autotuner (trial launches, cached per workload signature) instead of
+        the built-in heuristic; the choice and its table are in
+        ``self.tuned``."""
         self._lib = L.load()
+        self.tuned = None
+        shared = inst_conn is None or len(set(np.asarray(inst_conn).tolist())) == 1
+        if tune and not _dry and not tile and not waves and shared and len(conns) == 1 and \
+                kernel in (L.KERNEL_AUTO, L.KERNEL_BATCH_LANE) and np.asarray(params).shape[0] >= 32:
+            from . import autotune
+            kw = dict(model=model, cfun=cfun, scheme=scheme, precision=precision, noise=noise, conns=conns,
+                      params=params, dt=dt, obs_coef=obs_coef, monitors=monitors, inst_conn=inst_conn,
+                      node_params=node_params, node_param_mask=node_param_mask, seed=seed,
+                      inst_stream=inst_stream, stream=stream, flags=flags)
+            tile, waves, table = autotune.tune(kw, device=autotune.device_name())
+            kernel = L.KERNEL_BATCH_LANE
+            self.tuned = {"tile": tile, "waves": waves,
+                          "table_ms_per_step": {"%dx%d" % k: v for k, v in sorted(table.items())}}
         self.handle = None
         self.model, self.cfun, self.scheme = model, cfun, scheme
         self.precision, self.noise = precision, noise

@@ -145,7 +145,12 @@ struct tvbk_plan {
   size_t srec_w_off = 0, srec_rp_off = 0;   // weights' and row starts' word offsets in d_srec
   uint64_t par_umask = 0;                     // derived parameters shared by every instance
   double par_uval[TVBK_KMAX_PAR] = {0};
-  int mon_lds_off[TVBK_MAX_MON] = {0};
+  int mon_lds_off[TVBK_MAX_MON] = {0};       // image offsets (HBM, every node)
+  // the LDS image: nodes it holds, its offsets and bytes
+  int n_lds = 0, k_state = 0, k_mon_off[TVBK_MAX_MON] = {0};
+  int64_t k_img_bytes = 0;
+  bool batch_split = false;                  // batch-lane tiles split by node range
+  int batch_parts = 0;
   int mon_Q[TVBK_MAX_MON] = {0};
   // device buffers
   DevBuf d_tiles, d_conns, d_edges, d_w, d_rp, d_ring, d_img, d_bacc, d_params, d_flags, d_streams, d_z, d_srec;
@@ -293,28 +298,36 @@ int tvbk_plan_destroy(tvbk_plan* plan) {
 static int layout_lds(tvbk_plan* p, int T, int64_t max_lds, bool allow_csr) {
   const tvbk_plan_desc& d = p->d;
   const int N = p->max_n;
-  int64_t off = 0;
+  // tile image of n nodes, node-major regions: Bold interim sums, state,
+  // TemporalAverage sums; returns its bytes
+  auto image = [&](int n, int* mon_off, int* state_off) -> int64_t {
+    int64_t o = 0;
+    for (int m = 0; m < d.n_mon; ++m)
+      if (d.mon[m].kind == TVBK_MON_BOLD) {
+        mon_off[m] = (int)o;
+        o += (int64_t)n * d.mon[m].n_voi * T * 8;
+      }
+    *state_off = (int)o;
+    o += (int64_t)n * p->mi.nsv * T * p->esize;
+    for (int m = 0; m < d.n_mon; ++m)
+      if (d.mon[m].kind == TVBK_MON_TAVG) {
+        o = align_up(o, 8);
+        mon_off[m] = (int)o;
+        // float32: sum and Kahan compensation; float64: sum
+        o += (int64_t)n * d.mon[m].n_voi * (p->esize == 8 ? 1 : 2) * T * p->esize;
+      }
+    return align_up(o, 16);
+  };
   p->lds_bold = 0;
-  for (int m = 0; m < d.n_mon; ++m)
-    if (d.mon[m].kind == TVBK_MON_BOLD) {
-      p->mon_lds_off[m] = (int)off;
-      off += (int64_t)N * d.mon[m].n_voi * T * 8;
-    }
-  p->lds_state = (int)off;
-  off += (int64_t)N * p->mi.nsv * T * p->esize;
-  for (int m = 0; m < d.n_mon; ++m)
-    if (d.mon[m].kind == TVBK_MON_TAVG) {
-      off = align_up(off, 8);
-      p->mon_lds_off[m] = (int)off;
-      // float32: sum and Kahan compensation; float64: sum
-      off += (int64_t)N * d.mon[m].n_voi * (p->esize == 8 ? 1 : 2) * T * p->esize;
-    }
-  off = align_up(off, 16);
-  p->img_bytes = off;
+  // the HBM image holds every node; the LDS image the n_lds nodes of a
+  // tile's node range (all of them unless tiles are split by node range)
+  p->img_bytes = image(N, p->mon_lds_off, &p->lds_state);
+  int64_t off = image(p->n_lds, p->k_mon_off, &p->k_state);
+  p->k_img_bytes = off;
   p->lds_par = (int)off;                       // per-lane parameters [prow][T]
   off = align_up(off + (int64_t)p->prow * T * p->esize, 16);
   p->lds_c = (int)off;
-  off += (int64_t)std::max(2, p->tb_k) * N * p->nct * T * p->hsize;
+  off += (int64_t)std::max(2, p->tb_k) * p->n_lds * p->nct * T * p->hsize;
   off = align_up(off, 16);
   p->lds_si = (int)off;
   if (p->tb_k > 1) off = align_up(off + (int64_t)p->tb_k * TVBK_KMAX_MON * TVBK_STEPINFO_BYTES, 16);
@@ -490,6 +503,7 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   }
 
   // ---- kernel shape
+  p->n_lds = p->max_n;
   int dev = 0;
   HIPCHK(p, hipGetDevice(&dev));
   int max_lds = 65536;
@@ -553,17 +567,60 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
       }
     }
     if (T != 16 && T != 32 && T != 64) { p->err = "tile must be 16, 32 or 64"; return bail(TVBK_EINVAL); }
-    while (layout_lds(p, T, max_lds, true) != TVBK_OK) {
-      if (d.tile || T == 16) {
-        if (d.kernel == TVBK_KERNEL_AUTO && !d.tile) {   // large nodes: one instance per workgroup
-          kernel = p->kernel = TVBK_KERNEL_EDGE_LANE;
-          T = 1;
-          break;
-        }
+    bool fit = false;
+    while (true) {
+      if (layout_lds(p, T, max_lds, true) == TVBK_OK) { fit = true; break; }
+      if (d.tile || T == 16) break;
+      T /= 2;
+    }
+    // a tile whose nodes do not fit LDS is split by node range over several
+    // workgroups: each holds its range's state and coupling buffers, they
+    // step in blocks of K <= d_min steps (time-blocked loop), stage their
+    // history writes and a commit stores them between blocks.
+    // TVBK_BATCH_SPLIT: unset = when needed, 0 = never, n >= 2 = n parts
+    const char* envb = std::getenv("TVBK_BATCH_SPLIT");
+    const int forced_b = envb && *envb ? std::atoi(envb) : -1;
+    int dmin_all = p->conns[0].horizon;
+    for (const ConnDesc& cd : p->conns) dmin_all = std::min(dmin_all, cd.nnz > 0 ? cd.d_min : cd.horizon - 1);
+    const int N0 = p->conns[0].n_node;
+    if ((!fit || forced_b >= 2) && forced_b != 0 && !any_aff && dmin_all >= 2 && N0 >= 16) {
+      const int Ts = d.tile ? d.tile : (p->nch == 1 ? 32 : 16);     // n_channel * T == 32: whole-wave rows
+      const bool uni = (p->nch * Ts) % 32 == 0;
+      p->tb_k = std::min(std::min(dmin_all, (int)TVBK_TB_MAX), 16);
+      int P = forced_b >= 2 ? forced_b : 2;
+      bool ok = false;
+      for (; P <= 64 && P <= N0 / 8; ++P) {
+        p->n_lds = (N0 + P - 1) / P;
+        if (layout_lds(p, Ts, max_lds, !uni) == TVBK_OK) { ok = true; break; }
+        if (forced_b >= 2) break;
+      }
+      if (ok) {
+        // at least two workgroups per CU when the batch has few tiles
+        int n_cu = 256;
+        hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+        const int ntile0 = (d.n_inst + Ts - 1) / Ts;
+        if (forced_b < 2) P = std::max(P, std::min((2 * n_cu + ntile0 - 1) / ntile0, std::min(64, N0 / 8)));
+        p->n_lds = (N0 + P - 1) / P;
+        ok = layout_lds(p, Ts, max_lds, !uni) == TVBK_OK;
+      }
+      if (ok) {
+        p->batch_split = true;
+        p->batch_parts = P;
+        T = Ts;
+        fit = true;
+      } else {
+        p->n_lds = p->max_n;
+        p->tb_k = 0;
+      }
+    }
+    if (!fit) {
+      if (d.kernel == TVBK_KERNEL_AUTO && !d.tile) {   // large nodes: one instance per workgroup
+        kernel = p->kernel = TVBK_KERNEL_EDGE_LANE;
+        T = 1;
+      } else {
         p->err = "tile does not fit in LDS";
         return bail(TVBK_EUNSUPPORTED);
       }
-      T /= 2;
     }
   }
   if (kernel == TVBK_KERNEL_EDGE_LANE) {
@@ -575,6 +632,7 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   // scalar cache (no CSR in LDS)
   p->uniform_rows = kernel == TVBK_KERNEL_BATCH_LANE && (p->nch * T) % 32 == 0;
   if (p->uniform_rows) layout_lds(p, T, max_lds, false);
+  if (p->batch_split && p->lds_total > max_lds) { p->err = "split tile does not fit in LDS"; return bail(TVBK_EUNSUPPORTED); }
   {
     const int P = p->max_n * std::max(p->nch, 1);
     int nw;
@@ -652,6 +710,31 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
       }
   }
 
+  // ---- batch-lane tiles split by node range: parts of equal node counts
+  // (the LDS image holds n_lds nodes), sharing the tile's ring, image,
+  // accumulators and outputs; history writes staged per block of split_k
+  // steps
+  int64_t stage_len = 0;
+  if (p->batch_split) {
+    const int N0 = p->conns[0].n_node, P = p->batch_parts, nn = p->n_lds;
+    p->split_k = p->tb_k;
+    std::vector<TileDesc> extra;
+    for (int k = 0; k < ntiles; ++k) {
+      TileDesc& td = p->tiles[k];
+      td.parts = P;
+      td.node0 = 0;
+      td.node1 = std::min(nn, N0);
+      td.stage_off = stage_len;
+      stage_len += (int64_t)p->split_k * N0 * p->nch * T;
+      for (int q = 1; q < P; ++q) {
+        TileDesc t2 = td;
+        t2.node0 = q * nn;
+        t2.node1 = std::min((q + 1) * nn, N0);
+        if (t2.node1 > t2.node0) extra.push_back(t2);
+      }
+    }
+    p->tiles.insert(p->tiles.end(), extra.begin(), extra.end());
+  }
   // ---- edge-lane kernel: split heavy instances over several tiles (node
   // ranges with balanced edge counts), so that one large connectome does not
   // hold one CU for the whole call while the others idle.  Split tiles step
@@ -659,7 +742,6 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   // it, each tile stages its nodes' history writes, and a commit between
   // blocks stores them in the ring.  TVBK_SPLIT: unset = automatic, 0 = off,
   // n > 1 = n parts for every instance that can be split.
-  int64_t stage_len = 0;
   if (T == 1 && kernel == TVBK_KERNEL_EDGE_LANE && B > 0) {
     const char* env = std::getenv("TVBK_SPLIT");
     const int forced = env ? std::atoi(env) : -1;
@@ -1382,7 +1464,9 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
   a.dt = d.dt;
   a.sqrt_dt = std::sqrt(d.dt);
   a.lds_bold = p->lds_bold;
-  a.lds_state = p->lds_state;
+  a.lds_state = p->k_state;
+  a.lds_nodes = p->n_lds;
+  a.no_pipe = p->batch_split ? 1 : 0;
   a.lds_c = p->lds_c;
   a.lds_si = p->lds_si;
   a.tb_k = p->tb_k;
@@ -1418,24 +1502,27 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
     ma.taps = (const double*)p->d_taps[m].p;
     ma.stock = (double*)p->d_bstock[m].p;
     ma.src = p->src[m];
-    ma.lds_off = p->mon_lds_off[m];
+    ma.lds_off = p->k_mon_off[m];
   }
   a.dense_w = (const float*)p->d_dense.p;
   a.dense_kb = p->dense_kb;
   a.dense_rb = p->dense_rb;
   a.lds_x = p->lds_x;
-  // image regions a split tile writes back (node-major, bytes per node)
+  // image regions (node-major): HBM and LDS offsets, bytes per node
   a.n_img_reg = 0;
   for (int m = 0; m < d.n_mon; ++m)
     if (d.mon[m].kind == TVBK_MON_BOLD) {
       a.img_reg_off[a.n_img_reg] = p->mon_lds_off[m];
+      a.img_reg_lds[a.n_img_reg] = p->k_mon_off[m];
       a.img_reg_bpn[a.n_img_reg++] = d.mon[m].n_voi * T * 8;
     }
   a.img_reg_off[a.n_img_reg] = p->lds_state;
+  a.img_reg_lds[a.n_img_reg] = p->k_state;
   a.img_reg_bpn[a.n_img_reg++] = p->mi.nsv * T * p->esize;
   for (int m = 0; m < d.n_mon; ++m)
     if (d.mon[m].kind == TVBK_MON_TAVG) {
       a.img_reg_off[a.n_img_reg] = p->mon_lds_off[m];
+      a.img_reg_lds[a.n_img_reg] = p->k_mon_off[m];
       a.img_reg_bpn[a.n_img_reg++] = d.mon[m].n_voi * (p->esize == 8 ? 1 : 2) * T * p->esize;
     }
   a.stage = p->d_stage.p;

@@ -129,13 +129,17 @@ struct StepArgs {
   // 0 and the call's length otherwise); history staging of split tiles
   int32_t k0, k1;
   void* stage;
-  // image regions (byte offset, bytes per node) a split tile writes back
-  int32_t n_img_reg, pad_reg;
-  int32_t img_reg_off[1 + TVBK_KMAX_MON], img_reg_bpn[1 + TVBK_KMAX_MON];
+  // regions of the tile image, node-major: byte offset in the HBM image
+  // (every node), byte offset in the LDS image (the tile's node range from
+  // position 0) and bytes per node; nodes held by the LDS arrays
+  int32_t n_img_reg, lds_nodes;
+  int32_t img_reg_off[1 + TVBK_KMAX_MON], img_reg_lds[1 + TVBK_KMAX_MON], img_reg_bpn[1 + TVBK_KMAX_MON];
   MonArgs mon[TVBK_KMAX_MON];
   // time-blocked sequential loop: steps per block (<= 1: off) and the LDS
   // offset of its monitor step records [tb_k][TVBK_KMAX_MON]
   int32_t tb_k, lds_si;
+  // batch-lane tiles split by node range: the pipelined loop is not used
+  int32_t no_pipe, pad_np;
 };
 
 // Node -> area reduction of one monitor's subsampled output (GlobalAverage,

@@ -153,8 +153,8 @@ __device__ inline void stage_params(const StepArgs& a, unsigned char* smem, int 
 // s + 1 at step s, simulator.py:311-316).  EXT: afferent monitors may be
 // present; aff_only: the pass after the last step, afferent monitors only.
 template <class R, class M, int CF, bool EXT>
-__device__ __forceinline__ void monitors_node(const StepArgs& a, const KCfg& kc, unsigned char* smem, int i, int t,
-                                              int T, int N, bool live, int k, const StepInfo* sir, const R* x,
+__device__ __forceinline__ void monitors_node(const StepArgs& a, const KCfg& kc, unsigned char* smem, int i, int li,
+                                              int t, int T, int N, bool live, int k, const StepInfo* sir, const R* x,
                                               const R* c, bool aff_only) {
   using C = Cfun<M, CF>;
   constexpr int NSV = M::NSV, NCT = C::NCT;
@@ -217,7 +217,7 @@ __device__ __forceinline__ void monitors_node(const StepArgs& a, const KCfg& kc,
           cm = R(0);
         }
       };
-      auto slot_of = [&](int v, int q) -> R* { return ts + ((i * nv + v) * NSL + q) * T + t; };
+      auto slot_of = [&](int v, int q) -> R* { return ts + ((li * nv + v) * NSL + q) * T + t; };
       int v0 = 0;
       if (mo.kind == MON_RAW) {
         for (int v = 0; v < nv; ++v)
@@ -297,7 +297,7 @@ __device__ __forceinline__ void monitors_node(const StepArgs& a, const KCfg& kc,
           *ap = 0.0;
         }
       };
-      auto bslot = [&](int v) -> double* { return bi + (i * nv + v) * T + t; };
+      auto bslot = [&](int v) -> double* { return bi + (li * nv + v) * T + t; };
       for (int v = 0; v < nv; ++v) {
         double bs = *bslot(v);
         bold_value(v, sample_val(mo, v), bs);
@@ -315,7 +315,7 @@ __device__ __forceinline__ void monitors_node(const StepArgs& a, const KCfg& kc,
 // already (else they are loaded here).
 template <class R, class M, int CF, class NR, bool UM = false, bool EXT = false, class SINK>
 __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, unsigned char* smem, R* s_x, int i,
-                                            int t, int T, int N, bool live, int inst, uint64_t stream, int k,
+                                            int li, int t, int T, int N, bool live, int inst, uint64_t stream, int k,
                                             int64_t s, const StepInfo* sir, const R* par, int pst, const R* c,
                                             const double* zpre, int& bad, SINK sink) {
   using C = Cfun<M, CF>;
@@ -326,7 +326,7 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
   const int64_t spatial_row = (int64_t)N * a.prow;
   R x[NSV], xi[NSV];
 #pragma unroll
-  for (int v = 0; v < NSV; ++v) x[v] = s_x[(i * NSV + v) * T + t];
+  for (int v = 0; v < NSV; ++v) x[v] = s_x[(li * NSV + v) * T + t];
   // this lane's parameters (registers, stride 1, or LDS, stride T): model,
   // coupling, noise gfun
   R pp[NDP > 0 ? NDP : 1], cp[NCP > 0 ? NCP : 1], gf[NSV];
@@ -377,12 +377,12 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
 #pragma unroll
   for (int v = 0; v < NSV; ++v) {
     bad |= !isfinite(x[v]);
-    s_x[(i * NSV + v) * T + t] = x[v];
+    s_x[(li * NSV + v) * T + t] = x[v];
   }
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) sink(ch, C::template channel<R>(x, ch, cp));
 
-  monitors_node<R, M, CF, EXT>(a, kc, smem, i, t, T, N, live, k, sir, x, c, false);
+  monitors_node<R, M, CF, EXT>(a, kc, smem, i, li, t, T, N, live, k, sir, x, c, false);
 }
 
 // RH: history / weight / coupling-sum type, R: state, integration and
@@ -487,7 +487,15 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     for (int k = tid; k <= N; k += nthr) l_rp[k] = g_rp[k];
   }
   stage_params<R>(a, smem, td_inst0, td_n_inst, N);
-  copy_bytes(img, a.img + td_img_off, a.img_bytes);
+  // the tile image's regions (node-major), this tile's node range: from
+  // the HBM image (every node) to the LDS image (the range from position 0)
+  for (int r = 0; r < a.n_img_reg; ++r) {
+    const int64_t nw = (int64_t)(node1 - node0) * a.img_reg_bpn[r] / 4;
+    const uint32_t* sr = reinterpret_cast<const uint32_t*>(a.img + td_img_off + a.img_reg_off[r] +
+                                                           (int64_t)node0 * a.img_reg_bpn[r]);
+    uint32_t* d = reinterpret_cast<uint32_t*>(img + a.img_reg_lds[r]);
+    for (int64_t q = tid; q < nw; q += nthr) d[q] = sr[q];
+  }
   __syncthreads();
 
   // ---- per-lane instance parameters (phase B lane and phase A lane agree:
@@ -543,7 +551,9 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   // items (node-channel pair q, step j), row-major so that the lanes of a
   // row read consecutive slots of each edge; step j's sums into
   // cbuf0 + j * cstep
-  const int64_t cstep = (int64_t)N * NCT * T;
+  // LDS arrays indexed by node hold the tile's node range [node0, node1):
+  // node i sits at position i - node0 of a.lds_nodes
+  const int64_t cstep = (int64_t)a.lds_nodes * NCT * T;
   auto phase_a = [&](auto getrec, const int32_t* RP, RH* cbuf0, int nk) {
     const int q0 = node0 * NCH, P = (node1 - node0) * NCH, tot = nk * P;
     for (int it = wave * NG + g; it - g < tot; it += nwave * NG) {
@@ -597,9 +607,9 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
         // the raw sum; the coupling's post runs in the state precision in
         // phase B (coupling.py:166-167: post of the float32 sum in float64)
         if (CF == CF_SIGMOID_JR) {
-          cbuf[(i * NCT) * T + t] = acc;
+          cbuf[((i - node0) * NCT) * T + t] = acc;
         } else {
-          cbuf[(i * NCT + ch) * T + t] = acc;
+          cbuf[((i - node0) * NCT + ch) * T + t] = acc;
         }
       }
     }
@@ -684,7 +694,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   auto row_store = [&](RH* cbuf, int i, int q, RH acc) {
     if (q < QN) {
       const int ch = q / T, tq = q - (q / T) * T;
-      cbuf[(i * NCT + (CF == CF_SIGMOID_JR ? 0 : ch)) * T + tq] = acc;
+      cbuf[((i - node0) * NCT + (CF == CF_SIGMOID_JR ? 0 : ch)) * T + tq] = acc;
     }
   };
   // the same for wave-uniform rows: items (step kk, node i), one per wave
@@ -732,15 +742,16 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     R cpv[NCP > 0 ? NCP : 1];
 #pragma unroll
     for (int v = 0; v < NCP; ++v) cpv[v] = lane_param<R>(a, par, pst, NDP + v);
+    const int li = i - node0;
 #pragma unroll
-    for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(i * NCT + v) * T + t], cpv);
+    for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(li * NCT + v) * T + t], cpv);
 #pragma unroll
     for (int v = NCT; v < C::NCD; ++v) c[v] = C::post(R(0), cpv);
     RH* wr = staged ? reinterpret_cast<RH*>(a.stage) + stage_off + ((int64_t)(k - kbeg) * N + i) * rstride + t
                     : ring + ring_index(T, H, N, NCH, wslot, i, 0, t);
     const int64_t wch = staged ? T : w_ch;
-    RH* st = cbuf + (i * NCT) * T + t;
-    update_node<R, M, CF, RH, UM, PIPE != 1>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s_, sir, par,
+    RH* st = cbuf + (li * NCT) * T + t;
+    update_node<R, M, CF, RH, UM, PIPE != 1>(a, kc, smem, s_x, i, li, t, T, N, live, inst, stream, k, s_, sir, par,
                                              pst, c, zpre, bad,
                                   [&](int ch, R v) {
                                     if (stage) st[ch * T] = (RH)v;
@@ -773,8 +784,8 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
     const uint32_t lane_off = lane < QN ? (uint32_t)lane * (uint32_t)sizeof(RH) : 0u;
     for (int k = 0; k < nsteps; ++k, ++s) {
       const int buf = k & 1;
-      RH* cbuf = s_c + (int64_t)buf * N * NCT * T;
-      RH* cnext = s_c + (int64_t)(buf ^ 1) * N * NCT * T;
+      RH* cbuf = s_c + (int64_t)buf * a.lds_nodes * NCT * T;
+      RH* cnext = s_c + (int64_t)(buf ^ 1) * a.lds_nodes * NCT * T;
       const int wslot = (smod + 1 == H) ? 0 : smod + 1;   // s % H
       const uint32_t base_n = __builtin_amdgcn_readfirstlane((uint32_t)wslot * sstride);   // (s+1-1) % H
       // history stores wait until the step's gathers are reduced: a wave's
@@ -858,7 +869,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   const int kend = kfin + (a.has_aff ? 1 : 0);
   for (int k = kbeg; k < kend; ++k, ++s) {
     const int buf = k & 1;
-    RH* cbuf = s_c + (int64_t)buf * N * NCT * T;
+    RH* cbuf = s_c + (int64_t)buf * a.lds_nodes * NCT * T;
     // ================= phase A: coupling of step s ========================
     if (a.uniform_rows) phase_a_uni(cbuf, 1);
     else if (a.csr_in_lds) phase_a(rec_lds, l_rp, cbuf, 1);
@@ -870,13 +881,14 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
         R cpv[NCP > 0 ? NCP : 1];
 #pragma unroll
         for (int v = 0; v < NCP; ++v) cpv[v] = lane_param<R>(a, par, pst, NDP + v);
+        const int li = i - node0;
 #pragma unroll
-        for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(i * NCT + v) * T + t], cpv);
+        for (int v = 0; v < NCT; ++v) c[v] = C::post((R)cbuf[(li * NCT + v) * T + t], cpv);
 #pragma unroll
         for (int v = NCT; v < C::NCD; ++v) c[v] = C::post(R(0), cpv);
 #pragma unroll
-        for (int v = 0; v < NSV; ++v) x[v] = s_x[(i * NSV + v) * T + t];
-        monitors_node<R, M, CF, true>(a, kc, smem, i, t, T, N, live, kfin, kc.si[buf], x, c, true);
+        for (int v = 0; v < NSV; ++v) x[v] = s_x[(li * NSV + v) * T + t];
+        monitors_node<R, M, CF, true>(a, kc, smem, i, li, t, T, N, live, kfin, kc.si[buf], x, c, true);
       }
       break;
     }
@@ -892,17 +904,12 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   }
   }
   __syncthreads();
-  if (node0 == 0 && node1 == N) {
-    copy_bytes(a.img + td_img_off, img, a.img_bytes);
-  } else {
-    // a split tile writes back its own nodes' part of every region
-    for (int r = 0; r < a.n_img_reg; ++r) {
-      const int64_t o = a.img_reg_off[r] + (int64_t)node0 * a.img_reg_bpn[r];
-      const int64_t nw = (int64_t)(node1 - node0) * a.img_reg_bpn[r] / 4;
-      uint32_t* d = reinterpret_cast<uint32_t*>(a.img + td_img_off + o);
-      const uint32_t* sr = reinterpret_cast<const uint32_t*>(img + o);
-      for (int64_t q = tid; q < nw; q += nthr) d[q] = sr[q];
-    }
+  // write back this tile's nodes of every region
+  for (int r = 0; r < a.n_img_reg; ++r) {
+    const int64_t nw = (int64_t)(node1 - node0) * a.img_reg_bpn[r] / 4;
+    uint32_t* d = reinterpret_cast<uint32_t*>(a.img + td_img_off + a.img_reg_off[r] + (int64_t)node0 * a.img_reg_bpn[r]);
+    const uint32_t* sr = reinterpret_cast<const uint32_t*>(img + a.img_reg_lds[r]);
+    for (int64_t q = tid; q < nw; q += nthr) d[q] = sr[q];
   }
   if (bad && live) atomicOr(a.flags + inst, 1);
 }
@@ -1021,7 +1028,7 @@ __global__ void __launch_bounds__(1024) k_dense(StepArgs a) {
         for (int u = 0; u < NCP; ++u) cpv[u] = lane_param<R>(a, par, pst, NDP + u);
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch) c[ch] = C::post((R)d[ch][v], cpv);
-        update_node<R, M, CF, RH>(a, kc, smem, s_x, i, t, T, N, live, inst, stream, k, s, kc.si[buf], par, pst, c,
+        update_node<R, M, CF, RH>(a, kc, smem, s_x, i, i, t, T, N, live, inst, stream, k, s, kc.si[buf], par, pst, c,
                                   nullptr, bad,
                                   [&](int ch, R val) { Xn[((int64_t)ch * KP + i) * T + t] = (RH)val; });
       }
@@ -1088,7 +1095,8 @@ __global__ void k_init_uniform(InitArgs a) {
 template <class RH, class R, class M, int CF>
 hipError_t launch_step_t(const StepArgs& a, int ntiles, int nthreads, size_t lds, hipStream_t st) {
   // stimulus and afferent monitors run in the sequential loop only
-  const bool pipe = a.uniform_rows && !a.zero_delay && Cfun<M, CF>::NCH * a.T <= 64 && !a.stim_tp && !a.has_aff;
+  const bool pipe = a.uniform_rows && !a.zero_delay && Cfun<M, CF>::NCH * a.T <= 64 && !a.stim_tp && !a.has_aff &&
+                    !a.no_pipe;
   const uint64_t mmask = M::NDP >= 64 ? ~0ull : ((1ull << M::NDP) - 1);
   const bool um = pipe && (a.par_umask & mmask) == mmask && !a.spatial;
   // the time-blocked sequential loop: blocks of tb_k <= d_min steps, no

@@ -2,9 +2,11 @@
 connectome runs on several workgroups, each holding its node range in LDS,
 stepping in time blocks with staged history commits.
 
-* forced on a sweep that fits unsplit (TVBK_BATCH_SPLIT=3): bit-identical
-  to the unsplit plan, over several calls, in both precisions and with the
-  V-channel skip;
+* forced on a sweep that fits unsplit (TVBK_BATCH_SPLIT=3, against =0):
+  bit-identical to the unsplit plan, over several calls, in both
+  precisions and with the V-channel skip;
+* automatic on a sweep of few tiles (64 instances): split, and
+  bit-identical to the unsplit plan;
 * automatic on a 400-node connectome whose tiles do not fit LDS: sampled
   instances bit-exact against the reference-faithful oracle fed the device's
   normals, and every instance bit-identical to the edge-lane kernel."""
@@ -37,7 +39,7 @@ def test_forced_batch_split_equals_unsplit(prec, skip):
     w, Lm, d, H, G, sig = bench.c2_setup(256)
     G, sig = G[::4][:64], sig[::4][:64]
     outs = []
-    for env in (None, "3"):
+    for env in ("0", "3", None):
         old = _env("TVBK_BATCH_SPLIT", env)
         try:
             plan = bench.build_c2_plan(w, d, G, sig, 0.05, 900, P, L.NOISE_PHILOX, 3, bold=20.0, skip=skip,
@@ -48,7 +50,7 @@ def test_forced_batch_split_equals_unsplit(prec, skip):
             plan.init_uniform(11, [0.0, -2.0], [2.0, 1.5])
             ntile, k = plan.tiles()
             _say(prec, "skip" if skip else "", "split env", env, "tiles", ntile, "block", k, "tile", plan.tile)
-            assert (k > 0) == (env is not None)
+            assert (k > 0) == (env != "0")
             res = []
             for n in (900, 900, 450):
                 plan.run(n)
@@ -56,9 +58,10 @@ def test_forced_batch_split_equals_unsplit(prec, skip):
             outs.append(res)
         finally:
             plan.close()
-    for ca, cb in zip(*outs):
-        for a, b in zip(ca, cb):
-            np.testing.assert_array_equal(a, b)
+    for other in outs[1:]:
+        for ca, cb in zip(outs[0], other):
+            for a, b in zip(ca, cb):
+                np.testing.assert_array_equal(a, b)
 
 
 def test_large_connectome_batch_split_matches_oracle_and_edge_lane():

@@ -573,44 +573,72 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
       if (d.tile || T == 16) break;
       T /= 2;
     }
-    // a tile whose nodes do not fit LDS is split by node range over several
-    // workgroups: each holds its range's state and coupling buffers, they
-    // step in blocks of K <= d_min steps (time-blocked loop), stage their
-    // history writes and a commit stores them between blocks.
+    // a tile whose nodes do not fit LDS, or a batch of so few tiles that
+    // they fill at most half of the device's CUs, is split by node range
+    // over several workgroups: each holds its range's state and coupling
+    // buffers, they step in blocks of K <= d_min steps (time-blocked loop),
+    // stage their history writes and a commit stores them between blocks.
     // TVBK_BATCH_SPLIT: unset = when needed, 0 = never, n >= 2 = n parts
     const char* envb = std::getenv("TVBK_BATCH_SPLIT");
     const int forced_b = envb && *envb ? std::atoi(envb) : -1;
     int dmin_all = p->conns[0].horizon;
     for (const ConnDesc& cd : p->conns) dmin_all = std::min(dmin_all, cd.nnz > 0 ? cd.d_min : cd.horizon - 1);
     const int N0 = p->conns[0].n_node;
-    if ((!fit || forced_b >= 2) && forced_b != 0 && !any_aff && dmin_all >= 2 && N0 >= 16) {
-      const int Ts = d.tile ? d.tile : (p->nch == 1 ? 32 : 16);     // n_channel * T == 32: whole-wave rows
-      const bool uni = (p->nch * Ts) % 32 == 0;
+    int n_cu = 256;
+    hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+    n_cu = std::max(n_cu, 1);
+    const bool few = fit && !d.tile && 2 * ((d.n_inst + T - 1) / T) <= n_cu && N0 >= 32;
+    const int T_fit = T;
+    if ((!fit || few || forced_b >= 2) && forced_b != 0 && !any_aff && dmin_all >= 2 && N0 >= 16) {
       p->tb_k = std::min(std::min(dmin_all, (int)TVBK_TB_MAX), 16);
-      int P = forced_b >= 2 ? forced_b : 2;
-      bool ok = false;
-      for (; P <= 64 && P <= N0 / 8; ++P) {
-        p->n_lds = (N0 + P - 1) / P;
-        if (layout_lds(p, Ts, max_lds, !uni) == TVBK_OK) { ok = true; break; }
-        if (forced_b >= 2) break;
+      // candidate tiles: rows filling a whole wave (n_channel * T = 64), then
+      // half a wave; a given tile only
+      int cand[2], nc = 0;
+      if (d.tile) {
+        cand[nc++] = d.tile;
+      } else {
+        for (int q : {64, 32})
+          if (q % p->nch == 0 && (q / p->nch == 16 || q / p->nch == 32 || q / p->nch == 64)) cand[nc++] = q / p->nch;
+        if (nc == 0) cand[nc++] = 16;
       }
-      if (ok) {
-        // at least two workgroups per CU when the batch has few tiles
-        int n_cu = 256;
-        hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
-        const int ntile0 = (d.n_inst + Ts - 1) / Ts;
-        if (forced_b < 2) P = std::max(P, std::min((2 * n_cu + ntile0 - 1) / ntile0, std::min(64, N0 / 8)));
-        p->n_lds = (N0 + P - 1) / P;
-        ok = layout_lds(p, Ts, max_lds, !uni) == TVBK_OK;
+      const int pmax = std::min(64, N0 / 8);
+      int best_T = 0, best_P = 0;
+      double best_cost = 0.0;
+      for (int c = 0; c < nc; ++c) {
+        const int Tc = cand[c];
+        const bool uni = (p->nch * Tc) % 32 == 0;
+        // fewest parts whose node range fits LDS
+        int pmin = 0;
+        for (int P = forced_b >= 2 ? forced_b : 2; P <= pmax || (forced_b >= 2 && P == forced_b); ++P) {
+          p->n_lds = (N0 + P - 1) / P;
+          if (layout_lds(p, Tc, max_lds, !uni) == TVBK_OK) { pmin = P; break; }
+          if (forced_b >= 2) break;
+        }
+        if (pmin == 0) continue;
+        // workgroups run one per CU: take as many parts as fill the rounds
+        // that the fewest parts need, and rank tiles by rounds x rows per
+        // workgroup x waves per row
+        const int ntile0 = (d.n_inst + Tc - 1) / Tc;
+        const int64_t rounds = ((int64_t)ntile0 * pmin + n_cu - 1) / n_cu;
+        int P = pmin;
+        if (forced_b < 2) P = (int)std::max<int64_t>(pmin, std::min<int64_t>(pmax, rounds * n_cu / ntile0));
+        const double cost = (double)rounds * ((N0 + P - 1) / P) * (64.0 / std::min(64, p->nch * Tc));
+        if (best_T == 0 || cost < best_cost) { best_T = Tc; best_P = P; best_cost = cost; }
+      }
+      bool ok = false;
+      if (best_T) {
+        p->n_lds = (N0 + best_P - 1) / best_P;
+        ok = layout_lds(p, best_T, max_lds, (p->nch * best_T) % 32 != 0) == TVBK_OK;
       }
       if (ok) {
         p->batch_split = true;
-        p->batch_parts = P;
-        T = Ts;
+        p->batch_parts = best_P;
+        T = best_T;
         fit = true;
       } else {
         p->n_lds = p->max_n;
         p->tb_k = 0;
+        if (fit) layout_lds(p, T_fit, max_lds, true);   // the unsplit tile that fits
       }
     }
     if (!fit) {

@@ -178,12 +178,14 @@ def _concat_chunks(chunks, n_inst, n_mon):
 
 def synthetic_requests(n_requests, rate_hz, seed=0, inst_range=(1, 512), node_range=(68, 998),
                        duration_range_ms=(5000.0, 60000.0), duration_scale=1.0, models=("G2D", "MPR"),
-                       precision="mixed"):
+                       precision="mixed", skip_zero_coupling=True):
     """Poisson stream of sweep requests over seeded synthetic connectomes:
     N log-uniform in node_range, instances uniform in inst_range, model
     chosen uniformly, HeunStochastic, TemporalAverage(1 ms), per-request
     noise seed.  ``duration_scale`` shrinks the simulated durations (the
-    benchmark states the factor)."""
+    benchmark states the factor).  ``skip_zero_coupling``: MontbrioPazoRoxin
+    requests (cv = 0) keep only the r channel in their history ring
+    (TVBK_PLAN_SKIP_ZERO_COUPLING)."""
     from . import _lib as L
     from .connectome import idelays, synthetic
     from .plan import CSR, MonitorSpec, Plan
@@ -198,11 +200,11 @@ def synthetic_requests(n_requests, rate_hz, seed=0, inst_range=(1, 512), node_ra
         dur = rng.uniform(*duration_range_ms) * duration_scale
         cseed = int(rng.integers(1 << 30))
         nseed = int(rng.integers(1 << 62))
-        reqs.append(_make_request(k, t, n, B, model, dur, cseed, nseed, precision))
+        reqs.append(_make_request(k, t, n, B, model, dur, cseed, nseed, precision, skip_zero_coupling))
     return reqs
 
 
-def _make_request(rid, arrival, n, B, model, dur_ms, cseed, nseed, precision):
+def _make_request(rid, arrival, n, B, model, dur_ms, cseed, nseed, precision, skip_zero_coupling=False):
     from . import _lib as L
     from .connectome import idelays, synthetic
     from .plan import CSR, MonitorSpec, Plan
@@ -229,7 +231,8 @@ def _make_request(rid, arrival, n, B, model, dur_ms, cseed, nseed, precision):
     mons = [MonitorSpec(L.MON_TAVG, int(round(1.0 / dt)), [0], name="TemporalAverage")]
     obs = np.eye(2)[:1] if model == "G2D" else np.eye(2)[:1]
     kw = dict(model=mid, cfun=L.CFUN_LINEAR, scheme=L.HEUN_STO, precision=prec, noise=L.NOISE_PHILOX,
-              conns=[csr], params=params, dt=dt, obs_coef=obs, monitors=mons, seed=nseed)
+              conns=[csr], params=params, dt=dt, obs_coef=obs, monitors=mons, seed=nseed,
+              flags=L.PLAN_SKIP_ZERO_COUPLING if skip_zero_coupling and model == "MPR" else 0)
 
     def footprint(max_steps):
         return Plan.footprint(max_steps=max_steps, **kw)

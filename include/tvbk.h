@@ -113,6 +113,14 @@ enum {
   TVBK_KERNEL_DENSE_MFMA = 3  /* all delays zero; coupling W.X on MFMA      */
 };
 
+/* Step loops of the batch-lane and edge-lane kernels (tvbk_plan_loop). */
+enum {
+  TVBK_LOOP_SEQUENTIAL = 0,   /* coupling, then integration, one step at a time      */
+  TVBK_LOOP_PIPELINED = 1,    /* the next step's gather overlaps this step's update  */
+  TVBK_LOOP_TIME_BLOCKED = 2, /* the coupling of up to d_min steps gathered at once  */
+  TVBK_LOOP_DENSE = 3         /* the dense zero-delay kernel                        */
+};
+
 #define TVBK_MAX_OBS 8
 #define TVBK_MAX_MON 4
 #define TVBK_MAX_SVAR 8
@@ -385,6 +393,13 @@ int tvbk_plan_graphs(tvbk_plan *plan, int64_t *captured, int64_t *launched);
  * counterpart (a property of this backend's plan). */
 int tvbk_plan_channels(tvbk_plan *plan, int32_t *n_channel);
 
+/* Step loop (TVBK_LOOP_*) of the plan's last tvbk_run_steps; -1 before the
+ * first.  The pipelined loops need wave-uniform rows (one shared
+ * connectome, n_channel * tile a multiple of 32), every delay >= 1 and no
+ * stimulus or afferent monitor.  No reference counterpart (a property of
+ * this backend's launch). */
+int tvbk_plan_loop(tvbk_plan *plan, int32_t *loop);
+
 /* Kernel family, tile and waves chosen for the plan; LDS bytes per block. */
 int tvbk_plan_shape(tvbk_plan *plan, int32_t *kernel, int32_t *tile,
                     int32_t *waves, int32_t *lds_bytes);
@@ -422,6 +437,8 @@ int tvbk_balloon(const void *x, int32_t x_elem_bytes, int64_t x_stride,
  * 64 GiB, allocated and freed here), 16 pieces in flight per wave, one
  * 16-wave workgroup per CU; with write = 1 one segment is stored per 16
  * segments read (1/17 of the bytes, about the share of the ring writes).
+ * piece_bytes = 128: each wave load reads two random 128-byte pieces (one
+ * per half wave), as the gather of a one-channel tile of 32 instances.
  * *bytes_per_s receives the read + write rate of the best of three
  * launches of about 2 GiB.  Synchronous (default stream). */
 int tvbk_probe_gather(int64_t bytes, int32_t piece_bytes, int32_t write,

@@ -67,6 +67,7 @@ def test_c2_full_shape_sampled_instances_bit_exact(skip):
         init = {b: plan.export(b, np.float32) for b in cand}
         _say("c2 full shape: run", n_steps, "steps, tile", plan.tile, "waves", plan.waves)
         plan.run(n_steps)
+        assert plan.loop() == "pipelined"
         (tav, bol) = (plan.read(0, cand), plan.read(1, cand))
         flags = plan.flags()
         _say("  non-finite instances:", int((flags != 0).sum()))
@@ -241,6 +242,7 @@ def test_c3_full_batch_split_unsplit_and_oracle():
         _say("c3: 1024 instances, tiles split/unsplit", ps.tiles(), pu.tiles(), "998-node instance", big)
         for p in (ps, pu):
             p.run(max_steps)
+        assert ps.loop() == pu.loop() == "time-blocked"
         for m in range(2):
             for (t1, x1), (t2, x2) in zip(ps.read(m), pu.read(m)):
                 np.testing.assert_array_equal(t1, t2)

@@ -142,6 +142,7 @@ struct tvbk_plan {
   int64_t img_bytes = 0;
   bool csr_in_lds = false;
   bool uniform_rows = false;   // batch-lane, n_channel * T a multiple of 32
+  int last_loop = -1;          // TVBK_LOOP_* of the last run
   size_t srec_w_off = 0, srec_rp_off = 0;   // weights' and row starts' word offsets in d_srec
   uint64_t par_umask = 0;                     // derived parameters shared by every instance
   double par_uval[TVBK_KMAX_PAR] = {0};
@@ -1142,6 +1143,12 @@ int tvbk_plan_time_block(tvbk_plan* p, int32_t* steps) {
   return TVBK_OK;
 }
 
+int tvbk_plan_loop(tvbk_plan* p, int32_t* loop) {
+  if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
+  if (loop) *loop = p->last_loop;
+  return TVBK_OK;
+}
+
 int tvbk_plan_tiles(tvbk_plan* p, int32_t* n_tiles, int32_t* split_k) {
   if (!p) return fail(nullptr, TVBK_EINVAL, "null plan");
   if (n_tiles) *n_tiles = (int32_t)p->tiles.size();
@@ -1554,6 +1561,7 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
       a.img_reg_bpn[a.n_img_reg++] = d.mon[m].n_voi * (p->esize == 8 ? 1 : 2) * T * p->esize;
     }
   a.stage = p->d_stage.p;
+  p->last_loop = p->kernel == TVBK_KERNEL_DENSE_MFMA ? TVBK_LOOP_DENSE : step_loop(a, p->nch);
   HIPCHK(p, hipEventRecord(p->ev0, p->stream));
   if (p->kernel == TVBK_KERNEL_DENSE_MFMA) {
     a.k0 = 0;
@@ -1716,9 +1724,10 @@ int tvbk_probe_gather(int64_t bytes, int32_t piece_bytes, int32_t write, double*
   if (!bytes_per_s) return fail(nullptr, TVBK_EINVAL, "null argument");
   if (bytes < ((int64_t)64 << 20) || bytes > ((int64_t)64 << 30))
     return fail(nullptr, TVBK_EINVAL, "probe buffer must hold 64 MiB to 64 GiB");
-  const int k = piece_bytes / 256;
-  if (piece_bytes % 256 || (k != 1 && k != 2 && k != 4 && k != 8 && k != 16))
-    return fail(nullptr, TVBK_EINVAL, "piece_bytes must be 256, 512, 1024, 2048 or 4096");
+  // k = 0: 128-byte pieces (two per wave load)
+  const int k = piece_bytes == 128 ? 0 : piece_bytes / 256;
+  if (piece_bytes != 128 && (piece_bytes % 256 || (k != 1 && k != 2 && k != 4 && k != 8 && k != 16)))
+    return fail(nullptr, TVBK_EINVAL, "piece_bytes must be 128, 256, 512, 1024, 2048 or 4096");
   if (write != 0 && write != 1) return fail(nullptr, TVBK_EINVAL, "write must be 0 or 1");
   int dev = 0, n_cu = 0;
   hipError_t e = hipGetDevice(&dev);
@@ -1754,7 +1763,7 @@ int tvbk_probe_gather(int64_t bytes, int32_t piece_bytes, int32_t write, double*
   if (out) (void)hipFree(out);
   if (e != hipSuccess) return fail(nullptr, TVBK_EHIP, std::string("probe: ") + hipGetErrorString(e));
   const double rd = (double)grid * 16 * iters * 16 * 256;
-  const double wr = write ? (double)grid * 16 * (iters / k) * k * 256 : 0.0;
+  const double wr = !write ? 0.0 : k == 0 ? (double)grid * 16 * iters * 256 : (double)grid * 16 * (iters / k) * k * 256;
   *bytes_per_s = (rd + wr) / (best * 1e-3);
   return TVBK_OK;
 }

@@ -163,6 +163,16 @@ struct InitArgs {
   double lo[8], hi[8];
 };
 
+// The step kernel's loop (TVBK_LOOP_*) for a launch with arguments a whose
+// coupling keeps nch history channels: stimulus and afferent monitors run
+// in the sequential loop only; the time-blocked loop needs blocks of >= 2
+// steps, no afferent monitor and no zero delay.
+inline int step_loop(const StepArgs& a, int nch) {
+  const bool pipe = a.uniform_rows && !a.zero_delay && nch * a.T <= 64 && !a.stim_tp && !a.has_aff && !a.no_pipe;
+  if (pipe) return TVBK_LOOP_PIPELINED;
+  return (a.tb_k > 1 && !a.has_aff && !a.zero_delay) ? TVBK_LOOP_TIME_BLOCKED : TVBK_LOOP_SEQUENTIAL;
+}
+
 hipError_t launch_step(int precision, int model, int cf, const StepArgs& a, int ntiles, int nthreads, size_t lds,
                        hipStream_t st);
 hipError_t launch_init(int precision, int model, int cf, const InitArgs& a, int ntiles, hipStream_t st);

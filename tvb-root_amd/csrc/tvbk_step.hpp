@@ -1094,16 +1094,14 @@ __global__ void k_init_uniform(InitArgs a) {
 // ---------------------------------------------------------------------------
 template <class RH, class R, class M, int CF>
 hipError_t launch_step_t(const StepArgs& a, int ntiles, int nthreads, size_t lds, hipStream_t st) {
-  // stimulus and afferent monitors run in the sequential loop only
-  const bool pipe = a.uniform_rows && !a.zero_delay && Cfun<M, CF>::NCH * a.T <= 64 && !a.stim_tp && !a.has_aff &&
-                    !a.no_pipe;
+  // the loop (step_loop); the pipelined one reads every model parameter from
+  // the kernel arguments when all are plan constants (UM)
+  const int loop = step_loop(a, Cfun<M, CF>::NCH);
+  const bool pipe = loop == TVBK_LOOP_PIPELINED;
   const uint64_t mmask = M::NDP >= 64 ? ~0ull : ((1ull << M::NDP) - 1);
   const bool um = pipe && (a.par_umask & mmask) == mmask && !a.spatial;
-  // the time-blocked sequential loop: blocks of tb_k <= d_min steps, no
-  // afferent monitor, no zero delay
-  const bool tb = !pipe && a.tb_k > 1 && !a.has_aff && !a.zero_delay;
   auto kern = pipe ? (um ? k_step<RH, R, M, CF, 1, true> : k_step<RH, R, M, CF, 1, false>)
-                   : tb ? k_step<RH, R, M, CF, 2, false> : k_step<RH, R, M, CF, 0, false>;
+                   : loop == TVBK_LOOP_TIME_BLOCKED ? k_step<RH, R, M, CF, 2, false> : k_step<RH, R, M, CF, 0, false>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;

@@ -27,6 +27,8 @@ NOISE_NONE, NOISE_INJECTED, NOISE_PHILOX = 0, 1, 2
 MON_RAW, MON_TAVG, MON_BOLD = 0, 1, 2
 MON_SUBSAMPLE, MON_GAVG, MON_SAVG, MON_AFF, MON_AFF_TAVG = 3, 4, 5, 6, 7
 KERNEL_AUTO, KERNEL_BATCH_LANE, KERNEL_EDGE_LANE, KERNEL_DENSE_MFMA = 0, 1, 2, 3
+# step loops (TVBK_LOOP_*)
+LOOP_NAMES = {0: "sequential", 1: "pipelined", 2: "time-blocked", 3: "dense"}
 PLAN_SKIP_ZERO_COUPLING = 1
 
 
@@ -100,6 +102,7 @@ def _bind(lib):
         "tvbk_plan_shape": ([P, P, P, P, P], C.c_int),
         "tvbk_plan_tiles": ([P, P, P], C.c_int),
         "tvbk_plan_time_block": ([P, P], C.c_int),
+        "tvbk_plan_loop": ([P, P], C.c_int),
         "tvbk_plan_channels": ([P, P], C.c_int),
         "tvbk_plan_graphs": ([P, P, P], C.c_int),
         "tvbk_last_run_ms": ([P, P], C.c_int),
@@ -122,7 +125,8 @@ EXPORTED = (
     "tvbk_set_steps", "tvbk_set_stimulus", "tvbk_sync", "tvbk_monitor_layout", "tvbk_read_monitor",
     "tvbk_monitor_device_ptr", "tvbk_export_state", "tvbk_export_monitor", "tvbk_import_monitor",
     "tvbk_read_flags",
-    "tvbk_debug_slots", "tvbk_debug_normals", "tvbk_plan_shape", "tvbk_plan_tiles", "tvbk_plan_time_block", "tvbk_plan_channels", "tvbk_plan_graphs", "tvbk_last_run_ms",
+    "tvbk_debug_slots", "tvbk_debug_normals", "tvbk_plan_shape", "tvbk_plan_tiles", "tvbk_plan_time_block", "tvbk_plan_loop",
+    "tvbk_plan_channels", "tvbk_plan_graphs", "tvbk_last_run_ms",
     "tvbk_balloon",
     "tvbk_probe_gather", "tvbk_last_error")
 
@@ -172,7 +176,8 @@ def probe_gather(piece_bytes=256, write=False, nbytes=3200 << 20):
     """Read (+ write) bytes/s of the gather's access pattern on this device
     (tvbk_probe_gather): 256-byte pieces read in segments of ``piece_bytes``
     at random offsets of an ``nbytes`` buffer, with ``write`` one segment
-    stored per 16 read."""
+    stored per 16 read; ``piece_bytes`` 128: two random 128-byte pieces per
+    wave load."""
     lib = load()
     r = C.c_double(0.0)
     check(lib.tvbk_probe_gather(int(nbytes), int(piece_bytes), 1 if write else 0, C.byref(r)))

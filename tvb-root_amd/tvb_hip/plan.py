@@ -371,6 +371,12 @@ class Plan:
         self._chk(self._lib.tvbk_plan_time_block(self.handle, C.byref(k)))
         return k.value
 
+    def loop(self):
+        """Step loop of the last run (_lib.LOOP_NAMES), or None before it."""
+        k = C.c_int32()
+        self._chk(self._lib.tvbk_plan_loop(self.handle, C.byref(k)))
+        return L.LOOP_NAMES.get(k.value)
+
     def flags(self):
         f = np.zeros(self.n_inst, np.int32)
         self._chk(self._lib.tvbk_read_flags(self.handle, L.ptr(f)))

@@ -848,16 +848,21 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
       for (TileDesc& td : p->tiles) { td.parts = 1; td.node0 = 0; td.node1 = p->conns[td.conn].n_node; }
     }
   }
-  // a tile stages only its own rows of the CSR in LDS: with split instances
-  // the largest such range may fit where whole connectomes did not
+  // a tile stages only its own rows of the CSR and holds only its own node
+  // range in the LDS image and coupling buffers: with split instances the
+  // largest such range may fit where whole connectomes did not, and leaves
+  // room for longer time blocks
   if (kernel == TVBK_KERNEL_EDGE_LANE) {
     int64_t rn = 0;
+    int nr = 0;
     for (const TileDesc& td : p->tiles) {
       const tvbk_connectome& cn = d.conns[td.conn];
       rn = std::max<int64_t>(rn, (int64_t)cn.row_ptr[td.node1] - cn.row_ptr[td.node0]);
+      nr = std::max(nr, td.node1 - td.node0);
     }
-    if (rn < p->csr_nnz) {
-      p->csr_nnz = rn;
+    if (rn < p->csr_nnz || nr < p->n_lds) {
+      p->csr_nnz = std::min<int64_t>(p->csr_nnz, rn);
+      p->n_lds = std::min(p->n_lds, nr);
       layout_lds(p, 1, max_lds, true);
     }
   }
@@ -882,7 +887,13 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
         if (layout_lds(p, T, max_lds, allow) == TVBK_OK && p->csr_in_lds == csr0) break;
         p->tb_k = 0;
       }
-      if (!p->tb_k) layout_lds(p, T, max_lds, allow);
+      // split plans launch blocks of split_k steps: time blocks of equal
+      // length inside them (33 = one block of 33, not 32 + 1)
+      if (p->tb_k > 1 && p->split_k > p->tb_k) {
+        const int nb = (p->split_k + p->tb_k - 1) / p->tb_k;
+        p->tb_k = (p->split_k + nb - 1) / nb;
+      }
+      layout_lds(p, T, max_lds, allow);
     }
   }
   // a time block's gathers are (steps x rows) items: as many waves as the

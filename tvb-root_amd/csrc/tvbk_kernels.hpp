@@ -30,7 +30,7 @@ struct ConnDesc {
 #define TVBK_ROW_PAD 32
 // longest block of the time-blocked sequential loop, and the bytes of one
 // monitor step record (StepInfo) it keeps per step and monitor in LDS
-#define TVBK_TB_MAX 32
+#define TVBK_TB_MAX 64
 #define TVBK_STEPINFO_BYTES 48
 
 // Element index of history value (slot, node i, channel ch, lane t) in a

@@ -834,34 +834,52 @@ def run_copy(nbytes=2 << 30, reps=10):
             "launches": reps + 1}
 
 
-def run_c4(n_requests=16, rate_hz=None, duration_scale=1.0, inst_range=(1, 512), seed=404, est_rate=3e10,
-           busy=0.75, chunk_steps=20000):
+def run_c4(n_requests=16, rate_hz=None, duration_scale=1.0, inst_range=(1, 512), seed=404, busy=0.75,
+           chunk_steps=20000):
     """Serving: Poisson stream of sweep requests (G2D or MPR, HeunStochastic,
     N log-uniform in [68, 998], instances uniform in inst_range, durations
     U[5, 60] s x duration_scale), admitted by device footprint.  Without
-    ``rate_hz`` the arrival rate is set so that the offered work keeps the
-    GPU about ``busy`` occupied at ``est_rate`` region-steps/s.  Each
-    chunk's monitor samples stay on the device (copy-out is not in the
-    latency)."""
+    ``rate_hz`` the arrival rate is set from the device's serving capacity
+    for this mix -- the same requests arriving together, served first as a
+    burst (region-steps / makespan) -- so that the offered work keeps the
+    GPU about ``busy`` occupied.  Each chunk's monitor samples stay on the
+    device (copy-out is not in the latency)."""
     import torch
     from tvb_hip import _lib
     from tvb_hip.serve import Server, synthetic_requests
-    probe = synthetic_requests(n_requests, 1.0, seed=seed, inst_range=inst_range, duration_scale=duration_scale)
-    mean_rs = float(np.mean([r.region_steps for r in probe]))
-    if rate_hz is None:
-        rate_hz = busy * est_rate / mean_rs
-    reqs = synthetic_requests(n_requests, rate_hz, seed=seed, inst_range=inst_range, duration_scale=duration_scale)
     streams = [_lib.stream_create() for _ in range(4)]
     free, total = torch.cuda.mem_get_info()
-    srv = Server(hbm_budget=int(0.5 * free), streams=streams, chunk_steps=chunk_steps, keep_outputs=False, log=log)
-    stats = srv.run(reqs)
+
+    def serve(reqs):
+        srv = Server(hbm_budget=int(0.5 * free), streams=streams, chunk_steps=chunk_steps, keep_outputs=False,
+                     log=log)
+        return srv.run(reqs)
+
+    kw = dict(seed=seed, inst_range=inst_range, duration_scale=duration_scale)
+    calib = None
+    if rate_hz is None:
+        burst = synthetic_requests(n_requests, 1e9, **kw)       # the same requests, all at t = 0
+        st0 = serve(burst)
+        cap = st0["region_steps"] / st0["makespan_s"]
+        mean_rs = float(np.mean([r.region_steps for r in burst]))
+        rate_hz = busy * cap / mean_rs
+        calib = {"capacity_region_steps_per_s": cap, "burst_makespan_s": st0["makespan_s"],
+                 "burst_p50_latency_s": st0["p50_latency_s"], "burst_p99_latency_s": st0["p99_latency_s"]}
+    reqs = synthetic_requests(n_requests, rate_hz, **kw)
+    stats = serve(reqs)
     for st in streams:
         _lib.stream_destroy(st)
+    mean_rs = float(np.mean([r.region_steps for r in reqs]))
     stats.update(duration_scale=duration_scale, rate_hz=rate_hz, inst_range=list(inst_range),
-                 chunk_steps=chunk_steps, arrival_rate_basis="%.2f x %.3g region-steps/s over a mean request of "
-                 "%.3g region-steps" % (busy, est_rate, mean_rs),
+                 chunk_steps=chunk_steps,
                  requests=[dict(r.meta, rid=r.rid, latency_s=r.latency, region_steps=r.region_steps)
                            for r in reqs])
+    if calib:
+        stats.update(calib)
+        stats["offered_load"] = rate_hz * mean_rs / calib["capacity_region_steps_per_s"]
+        stats["arrival_rate_basis"] = ("%.2f x the serving capacity of this request mix (%.3g region-steps/s, the "
+                                       "same requests served as a burst) over a mean request of %.3g region-steps"
+                                       % (busy, calib["capacity_region_steps_per_s"], mean_rs))
     return stats
 
 
@@ -951,7 +969,7 @@ def main():
         "gather_pattern_ceiling": lambda: gather_pattern_ceiling(line),
         "single_instance_c1": lambda: run_c1(),
         "uneven_batch_c3": lambda: run_c3(stream),
-        "serving_c4": lambda: run_c4(est_rate=ctx["value"]),
+        "serving_c4": lambda: run_c4(),
         "zero_delay_mfma": lambda: run_c2_zero_delay(args.instances, stream=stream, precision=prec),
         "balloon_bold_fp64": lambda: run_balloon_on_c2(ctx),
         "cpu_baseline": lambda: cpu_baseline_dict(ctx, args.cpu_steps),

@@ -928,6 +928,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="the headline only")
     ap.add_argument("--extras", default=",".join(EXTRAS), help="comma-separated subset of: " + ",".join(EXTRAS))
     ap.add_argument("--workload", default="c2", choices=("c2", "c1", "c2z", "c3", "c4", "copy"))
+    ap.add_argument("--c3-scale", type=float, default=1.0,
+                    help="duration factor of the uneven batch for --workload c3 (1: the stated 2-30 s)")
     ap.add_argument("--precision", default="mixed", choices=("mixed", "f32"))
     ap.add_argument("--no-tune", action="store_true", help="the built-in kernel shape instead of the autotuner's")
     ap.add_argument("--all-channels", action="store_true",
@@ -950,7 +952,7 @@ def main():
     assert torch.cuda.is_available(), "no GPU"
     stream = torch.cuda.current_stream().cuda_stream
     if args.workload != "c2":
-        fn = {"c1": lambda: run_c1(), "c3": lambda: run_c3(stream),
+        fn = {"c1": lambda: run_c1(), "c3": lambda: run_c3(stream, duration_scale=args.c3_scale),
               "c2z": lambda: run_c2_zero_delay(args.instances, stream=stream,
                                                precision=L.MIXED if args.precision == "mixed" else L.F32),
               "copy": lambda: run_copy(), "c4": lambda: run_c4()}[args.workload]

@@ -118,7 +118,9 @@ enum {
   TVBK_LOOP_SEQUENTIAL = 0,   /* coupling, then integration, one step at a time      */
   TVBK_LOOP_PIPELINED = 1,    /* the next step's gather overlaps this step's update  */
   TVBK_LOOP_TIME_BLOCKED = 2, /* the coupling of up to d_min steps gathered at once  */
-  TVBK_LOOP_DENSE = 3         /* the dense zero-delay kernel                        */
+  TVBK_LOOP_DENSE = 3,        /* the dense zero-delay kernel                        */
+  TVBK_LOOP_PIPELINED2 = 4    /* pipelined, two steps per pass: one wave load       */
+                              /* gathers two consecutive steps of an edge           */
 };
 
 #define TVBK_MAX_OBS 8
@@ -396,8 +398,10 @@ int tvbk_plan_channels(tvbk_plan *plan, int32_t *n_channel);
 /* Step loop (TVBK_LOOP_*) of the plan's last tvbk_run_steps; -1 before the
  * first.  The pipelined loops need wave-uniform rows (one shared
  * connectome, n_channel * tile a multiple of 32), every delay >= 1 and no
- * stimulus or afferent monitor.  No reference counterpart (a property of
- * this backend's launch). */
+ * stimulus or afferent monitor; the two-step one rows of exactly 32 values,
+ * every delay >= 3 and its four coupling buffers in LDS (environment
+ * TVBK_PIPE2=0 at plan creation keeps the one-step loop).  No reference
+ * counterpart (a property of this backend's launch). */
 int tvbk_plan_loop(tvbk_plan *plan, int32_t *loop);
 
 /* Kernel family, tile and waves chosen for the plan; LDS bytes per block. */

@@ -139,6 +139,9 @@ struct tvbk_plan {
   // time-blocked sequential loop: steps per block (0: one step at a time)
   // and the LDS offset of the block's monitor step records
   int tb_k = 0, lds_si = 0;
+  // pipelined loop's steps per pass (2: TVBK_LOOP_PIPELINED2, whose rings
+  // carry one mirror slot past the horizon)
+  int pipe_k = 1;
   int64_t img_bytes = 0;
   bool csr_in_lds = false;
   bool uniform_rows = false;   // batch-lane, n_channel * T a multiple of 32
@@ -328,10 +331,13 @@ static int layout_lds(tvbk_plan* p, int T, int64_t max_lds, bool allow_csr) {
   p->lds_par = (int)off;                       // per-lane parameters [prow][T]
   off = align_up(off + (int64_t)p->prow * T * p->esize, 16);
   p->lds_c = (int)off;
-  off += (int64_t)std::max(2, p->tb_k) * p->n_lds * p->nct * T * p->hsize;
+  off += (int64_t)std::max(2 * p->pipe_k, p->tb_k) * p->n_lds * p->nct * T * p->hsize;
   off = align_up(off, 16);
   p->lds_si = (int)off;
-  if (p->tb_k > 1) off = align_up(off + (int64_t)p->tb_k * TVBK_KMAX_MON * TVBK_STEPINFO_BYTES, 16);
+  // monitor step records: one per step of a time block, or four (two passes
+  // of two steps) for the two-step pipelined loop
+  const int nsi = p->tb_k > 1 ? p->tb_k : (p->pipe_k == 2 ? 4 : 0);
+  if (nsi) off = align_up(off + (int64_t)nsi * TVBK_KMAX_MON * TVBK_STEPINFO_BYTES, 16);
   // CSR staged as 16-byte records {col offset, delay, weight bits}
   const int64_t ne = p->csr_nnz + TVBK_ROW_PAD;
   const int64_t csr = ne * 16 + align_up((N + 1) * 4, 16);
@@ -661,6 +667,24 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   // scalar cache (no CSR in LDS)
   p->uniform_rows = kernel == TVBK_KERNEL_BATCH_LANE && (p->nch * T) % 32 == 0;
   if (p->uniform_rows) layout_lds(p, T, max_lds, false);
+  // two-step pipelined loop: rows of 32 values (half a wave, so a wave load
+  // gathers two steps of an edge), every delay >= 3, no afferent monitor,
+  // history channels staged in the coupling buffers (as many as coupling
+  // terms); four coupling buffers and step records in LDS.  TVBK_PIPE2=0
+  // keeps the one-step loop.
+  p->pipe_k = 1;
+  if (p->uniform_rows && !p->batch_split && p->nch * T == 32 && p->nct == p->nch && !any_aff) {
+    int dmin2 = 1 << 30;
+    for (const ConnDesc& cd : p->conns) dmin2 = std::min(dmin2, cd.nnz > 0 ? cd.d_min : 0);
+    const char* envp = std::getenv("TVBK_PIPE2");
+    if (dmin2 >= 3 && !(envp && *envp && std::atoi(envp) == 0)) {
+      p->pipe_k = 2;
+      if (layout_lds(p, T, max_lds, false) != TVBK_OK) {
+        p->pipe_k = 1;
+        layout_lds(p, T, max_lds, false);
+      }
+    }
+  }
   if (p->batch_split && p->lds_total > max_lds) { p->err = "split tile does not fit in LDS"; return bail(TVBK_EUNSUPPORTED); }
   {
     const int P = p->max_n * std::max(p->nch, 1);
@@ -717,8 +741,9 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
       p->lane_of[b0 + l] = l;
     }
     td.ring_off = ring_off;
-    ring_off += (int64_t)cd.horizon * cd.n_node * p->nch * T;
-    if ((int64_t)cd.horizon * cd.n_node * p->nch * T * p->hsize >= (1LL << 31)) {
+    const int64_t hslots = cd.horizon + (p->pipe_k == 2 ? 1 : 0);   // + the mirror slot
+    ring_off += hslots * cd.n_node * p->nch * T;
+    if (hslots * cd.n_node * p->nch * T * p->hsize >= (1LL << 31)) {
       p->err = "tile history exceeds 2 GiB";
       return bail(TVBK_EUNSUPPORTED);
     }
@@ -1513,6 +1538,7 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
   a.lds_state = p->k_state;
   a.lds_nodes = p->n_lds;
   a.no_pipe = p->batch_split ? 1 : 0;
+  a.pipe_k = p->pipe_k;
   a.lds_c = p->lds_c;
   a.lds_si = p->lds_si;
   a.tb_k = p->tb_k;

@@ -138,8 +138,11 @@ struct StepArgs {
   // time-blocked sequential loop: steps per block (<= 1: off) and the LDS
   // offset of its monitor step records [tb_k][TVBK_KMAX_MON]
   int32_t tb_k, lds_si;
-  // batch-lane tiles split by node range: the pipelined loop is not used
-  int32_t no_pipe, pad_np;
+  // batch-lane tiles split by node range: the pipelined loop is not used.
+  // pipe_k = 2: the pipelined loop takes two steps per pass (rows of
+  // n_channel * T = 32 values, every delay >= 3, rings of H + 1 slots whose
+  // last mirrors slot 0)
+  int32_t no_pipe, pipe_k;
 };
 
 // Node -> area reduction of one monitor's subsampled output (GlobalAverage,
@@ -169,7 +172,7 @@ struct InitArgs {
 // steps, no afferent monitor and no zero delay.
 inline int step_loop(const StepArgs& a, int nch) {
   const bool pipe = a.uniform_rows && !a.zero_delay && nch * a.T <= 64 && !a.stim_tp && !a.has_aff && !a.no_pipe;
-  if (pipe) return TVBK_LOOP_PIPELINED;
+  if (pipe) return a.pipe_k == 2 ? TVBK_LOOP_PIPELINED2 : TVBK_LOOP_PIPELINED;
   return (a.tb_k > 1 && !a.has_aff && !a.zero_delay) ? TVBK_LOOP_TIME_BLOCKED : TVBK_LOOP_SEQUENTIAL;
 }
 

@@ -392,7 +392,9 @@ __device__ __forceinline__ void update_node(const StepArgs& a, const KCfg& kc, u
 // PIPE = 1: the pipelined loop only (wave-uniform rows, every delay >= 1,
 // n_channel * T <= 64); PIPE = 0: the sequential loop with every gather
 // variant; PIPE = 2: the time-blocked sequential loop (blocks of up to
-// d_min steps, no afferent monitor).  Separate instantiations keep each
+// d_min steps, no afferent monitor); PIPE = 3: the pipelined loop with two
+// steps per pass (rows of 32 values, every delay >= 3).  Separate
+// instantiations keep each
 // loop's register allocation to itself.  UM: every model parameter is a plan constant (a coupling or noise
 // sweep) and is read from the kernel arguments.
 template <class RH, class R, class M, int CF, int PIPE, bool UM>
@@ -537,7 +539,8 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   // the pipelined loop keeps two rows of RP registers in flight
   constexpr int RP = sizeof(RH) == 4 ? 24 : 12;
   using RPc = std::integral_constant<int, RP>;
-  const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(ring, (uint32_t)H * sstride);
+  // (the two-step pipelined loop's rings hold a mirror of slot 0 at slot H)
+  const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(ring, (uint32_t)(PIPE == 3 ? H + 1 : H) * sstride);
   const int rmax = cd_max_row < RMAX ? cd_max_row : RMAX;
   // byte offset of the slot `delay` steps before the step whose (s - 1) % H
   // is sm
@@ -751,7 +754,7 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
                     : ring + ring_index(T, H, N, NCH, wslot, i, 0, t);
     const int64_t wch = staged ? T : w_ch;
     RH* st = cbuf + (li * NCT) * T + t;
-    update_node<R, M, CF, RH, UM, PIPE != 1>(a, kc, smem, s_x, i, li, t, T, N, live, inst, stream, k, s_, sir, par,
+    update_node<R, M, CF, RH, UM, PIPE != 1 && PIPE != 3>(a, kc, smem, s_x, i, li, t, T, N, live, inst, stream, k, s_, sir, par,
                                              pst, c, zpre, bad,
                                   [&](int ch, R v) {
                                     if (stage) st[ch * T] = (RH)v;
@@ -810,6 +813,84 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
       if (tid == 0)
         for (int m = 0; m < a.n_mon; ++m) step_info(kc.mon[m], k + 1, kc.si[buf ^ 1][m]);
       smod = wslot;
+      __syncthreads();
+    }
+  } else if constexpr (PIPE == 3) {
+    // ---- two-step pipelined loop (uniform rows of QN = 32 values, every
+    // delay >= 3): a pass integrates steps s and s+1 while it gathers the
+    // coupling of steps s+2 and s+3, which read slots <= s-1 only.  One wave
+    // load serves both gathered steps of an edge: lanes 0..31 read the
+    // edge's slot for step s+2, lanes 32..63 the next slot (step s+3), so
+    // every lane carries a value, each edge's scalar offset serves two steps
+    // and the bytes in flight per row double.  The slot after H-1 is slot 0:
+    // the ring holds a mirror of slot 0 at slot H, refreshed here and
+    // whenever slot 0 is written.  Coupling buffers: two sets of two steps,
+    // alternating by pass; monitor step records [4][TVBK_KMAX_MON] in LDS.
+    StepInfo* si4 = reinterpret_cast<StepInfo*>(smem + a.lds_si);
+    RH* ring_h = ring + (int64_t)H * N * rstride;        // the mirror slot
+    for (int q = tid; q < N * rstride; q += nthr) ring_h[q] = ring[q];
+    if (tid == 0)
+      for (int m = 0; m < a.n_mon; ++m) {
+        si4[m] = kc.si[0][m];                           // step 0 (kbeg = 0)
+        step_info(kc.mon[m], 1, si4[TVBK_KMAX_MON + m]);
+      }
+    phase_a_uni(s_c, 2);                                // coupling of steps 0 and 1
+    __syncthreads();
+    const int w0 = __builtin_amdgcn_readfirstlane(nwave - 1 - wave);
+    const int nrow = (N - w0 + nwave - 1) / nwave;      // rows of this wave
+    const int npass = (N + nthr / T - 1) / (nthr / T);  // integration passes
+    const int nu = nrow > npass ? nrow : npass;
+    const int hl = lane >> 5, ql = lane & 31;           // step half, value of the row
+    const uint32_t lane_off = (uint32_t)ql * (uint32_t)sizeof(RH) + (uint32_t)hl * sstride;
+    // staged history of one step into slot wslot (and the mirror)
+    auto ring_out2 = [&](const RH* cbuf, int wslot) {
+      for (int i = tid / T; i < N; i += nthr / T) {
+        RH* wr = ring + (wslot * N + i) * rstride + t;
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+          const RH v = cbuf[(i * NCT + ch) * T + t];
+          wr[ch * T] = v;
+          if (wslot == 0) wr[(int64_t)H * N * rstride + ch * T] = v;
+        }
+      }
+    };
+    for (int k = 0; k < nsteps; k += 2, s += 2) {
+      const int set = (k >> 1) & 1;
+      RH* cb = s_c + (int64_t)set * 2 * cstep;           // steps s, s+1
+      RH* cn = s_c + (int64_t)(set ^ 1) * 2 * cstep;     // steps s+2, s+3
+      const StepInfo* sir = si4 + set * 2 * TVBK_KMAX_MON;
+      const bool two = k + 1 < nsteps;                   // uniform
+      const int ws0 = (smod + 1 == H) ? 0 : smod + 1;     // s % H
+      const int ws1 = (ws0 + 1 == H) ? 0 : ws0 + 1;       // (s+1) % H
+      const uint32_t base_n = __builtin_amdgcn_readfirstlane((uint32_t)ws1 * sstride);   // (s+2-1) % H
+      RH va[RP], vb[RP];
+      int ea = 0, na = 0, eb = 0, nb = 0;
+      if (nrow > 0) na = row_issue(RPc{}, w0, base_n, lane_off, va, ea);
+      auto body = [&](int u, RH* vf, int ef, int nf, RH* vi, int& ei, int& ni) {
+        if (u + 1 < nrow) ni = row_issue(RPc{}, w0 + (u + 1) * nwave, base_n, lane_off, vi, ei);
+        const int ib = tid / T + u * (nthr / T);
+        if (ib < N) {
+          b_item(ib, cb, k, s, sir, ws0, true, nullptr);
+          if (two) b_item(ib, cb + cstep, k + 1, s + 1, sir + TVBK_KMAX_MON, ws1, true, nullptr);
+        }
+        if (u < nrow) {
+          const RH acc = row_finish(RPc{}, ef, nf, base_n, lane_off, vf);
+          const int i = w0 + u * nwave, ch = ql / T, tq = ql - (ql / T) * T;
+          cn[hl * cstep + ((i - node0) * NCT + ch) * T + tq] = acc;
+        }
+      };
+      for (int u = 0; u < nu; u += 2) {
+        body(u, va, ea, na, vb, eb, nb);
+        if (u + 1 < nu) body(u + 1, vb, eb, nb, va, ea, na);
+      }
+      ring_out2(cb, ws0);
+      if (two) ring_out2(cb + cstep, ws1);
+      if (tid == 0)
+        for (int m = 0; m < a.n_mon; ++m) {
+          step_info(kc.mon[m], k + 2, si4[(set ^ 1) * 2 * TVBK_KMAX_MON + m]);
+          step_info(kc.mon[m], k + 3, si4[((set ^ 1) * 2 + 1) * TVBK_KMAX_MON + m]);
+        }
+      smod = two ? ws1 : ws0;
       __syncthreads();
     }
   } else if constexpr (PIPE == 2) {
@@ -1097,11 +1178,13 @@ hipError_t launch_step_t(const StepArgs& a, int ntiles, int nthreads, size_t lds
   // the loop (step_loop); the pipelined one reads every model parameter from
   // the kernel arguments when all are plan constants (UM)
   const int loop = step_loop(a, Cfun<M, CF>::NCH);
-  const bool pipe = loop == TVBK_LOOP_PIPELINED;
+  const bool pipe = loop == TVBK_LOOP_PIPELINED || loop == TVBK_LOOP_PIPELINED2;
   const uint64_t mmask = M::NDP >= 64 ? ~0ull : ((1ull << M::NDP) - 1);
   const bool um = pipe && (a.par_umask & mmask) == mmask && !a.spatial;
-  auto kern = pipe ? (um ? k_step<RH, R, M, CF, 1, true> : k_step<RH, R, M, CF, 1, false>)
-                   : loop == TVBK_LOOP_TIME_BLOCKED ? k_step<RH, R, M, CF, 2, false> : k_step<RH, R, M, CF, 0, false>;
+  const bool pipe2 = loop == TVBK_LOOP_PIPELINED2;
+  auto kern = pipe2 ? (um ? k_step<RH, R, M, CF, 3, true> : k_step<RH, R, M, CF, 3, false>)
+              : pipe ? (um ? k_step<RH, R, M, CF, 1, true> : k_step<RH, R, M, CF, 1, false>)
+                     : loop == TVBK_LOOP_TIME_BLOCKED ? k_step<RH, R, M, CF, 2, false> : k_step<RH, R, M, CF, 0, false>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;

@@ -28,7 +28,7 @@ MON_RAW, MON_TAVG, MON_BOLD = 0, 1, 2
 MON_SUBSAMPLE, MON_GAVG, MON_SAVG, MON_AFF, MON_AFF_TAVG = 3, 4, 5, 6, 7
 KERNEL_AUTO, KERNEL_BATCH_LANE, KERNEL_EDGE_LANE, KERNEL_DENSE_MFMA = 0, 1, 2, 3
 # step loops (TVBK_LOOP_*)
-LOOP_NAMES = {0: "sequential", 1: "pipelined", 2: "time-blocked", 3: "dense"}
+LOOP_NAMES = {0: "sequential", 1: "pipelined", 2: "time-blocked", 3: "dense", 4: "pipelined-2"}
 PLAN_SKIP_ZERO_COUPLING = 1
 
 

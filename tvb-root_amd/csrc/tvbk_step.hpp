@@ -604,8 +604,20 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
             const EdgeRec r = getrec(e0 + e);
             return buf_load<RH>(rsrc, e < n ? slot_off(sm, r.delay) + r.coff + lane_off : 0xFFFFFFF0u);
           };
+          // a block's eight addresses are formed (eight record reads, one
+          // wait) before its eight loads are issued
+          auto ld8 = [&](int e, RH* x) {
+            uint32_t o[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+              const EdgeRec r = getrec(e0 + e + k);
+              o[k] = e + k < n ? slot_off(sm, r.delay) + r.coff + lane_off : 0xFFFFFFF0u;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) x[k] = buf_load<RH>(rsrc, o[k]);
+          };
           auto wt = [&](int e) -> RH { return rec_weight<RH>(getrec(e0 + e)); };
-          acc = reduceat_row_blocked<RH, true>(n, ld, wt);
+          acc = reduceat_row_blocked8<RH, true>(n, ld, ld8, wt);
         }
         // the raw sum; the coupling's post runs in the state precision in
         // phase B (coupling.py:166-167: post of the float32 sum in float64)
@@ -870,8 +882,10 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
         if (u + 1 < nrow) ni = row_issue(RPc{}, w0 + (u + 1) * nwave, base_n, lane_off, vi, ei);
         const int ib = tid / T + u * (nthr / T);
         if (ib < N) {
-          b_item(ib, cb, k, s, sir, ws0, true, nullptr);
-          if (two) b_item(ib, cb + cstep, k + 1, s + 1, sir + TVBK_KMAX_MON, ws1, true, nullptr);
+          // steps s and s+1 of the item (one copy of the update in the code)
+#pragma unroll 1
+          for (int j = 0; j < (two ? 2 : 1); ++j)
+            b_item(ib, cb + j * cstep, k + j, s + j, sir + j * TVBK_KMAX_MON, j ? ws1 : ws0, true, nullptr);
         }
         if (u < nrow) {
           const RH acc = row_finish(RPc{}, ef, nf, base_n, lane_off, vf);
@@ -1181,10 +1195,14 @@ hipError_t launch_step_t(const StepArgs& a, int ntiles, int nthreads, size_t lds
   const bool pipe = loop == TVBK_LOOP_PIPELINED || loop == TVBK_LOOP_PIPELINED2;
   const uint64_t mmask = M::NDP >= 64 ? ~0ull : ((1ull << M::NDP) - 1);
   const bool um = pipe && (a.par_umask & mmask) == mmask && !a.spatial;
-  const bool pipe2 = loop == TVBK_LOOP_PIPELINED2;
-  auto kern = pipe2 ? (um ? k_step<RH, R, M, CF, 3, true> : k_step<RH, R, M, CF, 3, false>)
-              : pipe ? (um ? k_step<RH, R, M, CF, 1, true> : k_step<RH, R, M, CF, 1, false>)
-                     : loop == TVBK_LOOP_TIME_BLOCKED ? k_step<RH, R, M, CF, 2, false> : k_step<RH, R, M, CF, 0, false>;
+  // the two-step loop is built for models of up to two state variables
+  // whose model parameters are plan constants (the host plans it only for
+  // those)
+  auto kern = pipe ? (um ? k_step<RH, R, M, CF, 1, true> : k_step<RH, R, M, CF, 1, false>)
+                   : loop == TVBK_LOOP_TIME_BLOCKED ? k_step<RH, R, M, CF, 2, false> : k_step<RH, R, M, CF, 0, false>;
+  if constexpr (M::NSV <= 2) {
+    if (loop == TVBK_LOOP_PIPELINED2 && um) kern = k_step<RH, R, M, CF, 3, true>;
+  }
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;

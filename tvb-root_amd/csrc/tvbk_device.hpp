@@ -248,16 +248,17 @@ __device__ inline R reduceat_regs_w(int n, const R* v, const W* w) {
 
 // numpy add.reduceat order for rows longer than the register path: the
 // pairwise blocks of eight are loaded as a batch before they are used.
-// ld8(e, x) fills x[0..7] with the raw values of entries e..e+7 (zero past
-// the row end), wt(e) is entry e's weight; products are rounded before they
-// are summed, as numpy does.  PL: the next block's loads are issued before
-// the current block is summed (two blocks in flight, the sums in the same
-// order); every entry read lies below s + n + 8, as without it.
-template <class R, bool PL, class LD8, class WT>
-__device__ inline R pw_leaf_blocked(int s, int n, LD8& ld8, WT& wt) {
+// ld(e) returns the raw value of entry e (zero past the row end), wt(e) its
+// weight; products are rounded before they are summed, as numpy does.
+// PL: the next block's loads are issued before the current block is summed
+// (two blocks in flight, the sums in the same order); every entry read lies
+// below s + n + 8, as without it.
+template <class R, bool PL, class LD, class WT>
+__device__ inline R pw_leaf_blocked(int s, int n, LD& ld, WT& wt) {
   R x[8];
   if (n < 8) {
-    ld8(s, x);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = ld(s + k);
     R res = R(0);
 #pragma unroll
     for (int k = 0; k < 7; ++k)
@@ -266,14 +267,17 @@ __device__ inline R pw_leaf_blocked(int s, int n, LD8& ld8, WT& wt) {
   }
   if constexpr (PL) {
     R r[8], y[8];
-    ld8(s, x);
-    ld8(s + 8, y);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = ld(s + k);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) y[k] = ld(s + 8 + k);
 #pragma unroll
     for (int k = 0; k < 8; ++k) r[k] = wt(s + k) * x[k];
     const int nfull = n - (n & 7);
     int i = 8;
     for (; i < nfull; i += 8) {
-      ld8(s + i + 8, x);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) x[k] = ld(s + i + 8 + k);
 #pragma unroll
       for (int k = 0; k < 8; ++k) r[k] += wt(s + i + k) * y[k];
 #pragma unroll
@@ -287,59 +291,50 @@ __device__ inline R pw_leaf_blocked(int s, int n, LD8& ld8, WT& wt) {
     return res;
   }
   R r[8];
-  ld8(s, x);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[k] = ld(s + k);
 #pragma unroll
   for (int k = 0; k < 8; ++k) r[k] = wt(s + k) * x[k];
   const int nfull = n - (n & 7);
   int i = 8;
   for (; i < nfull; i += 8) {
-    ld8(s + i, x);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = ld(s + i + k);
 #pragma unroll
     for (int k = 0; k < 8; ++k) r[k] += wt(s + i + k) * x[k];
   }
   R res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
   const int rem = n - nfull;
-  ld8(s + i, x);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[k] = ld(s + i + k);
 #pragma unroll
   for (int k = 0; k < 7; ++k)
     if (k < rem) res += wt(s + i + k) * x[k];
   return res;
 }
 
-template <int DEPTH, class R, bool PL, class LD8, class WT>
-__device__ inline R pw_sum_blocked(int s, int n, LD8& ld8, WT& wt) {
+template <int DEPTH, class R, bool PL, class LD, class WT>
+__device__ inline R pw_sum_blocked(int s, int n, LD& ld, WT& wt) {
   if constexpr (DEPTH == 0) {
-    return pw_leaf_blocked<R, PL>(s, n, ld8, wt);
+    return pw_leaf_blocked<R, PL>(s, n, ld, wt);
   } else {
-    if (n <= 128) return pw_leaf_blocked<R, PL>(s, n, ld8, wt);
+    if (n <= 128) return pw_leaf_blocked<R, PL>(s, n, ld, wt);
     int n2 = n / 2;
     n2 -= n2 & 7;
-    R a = pw_sum_blocked<DEPTH - 1, R, PL>(s, n2, ld8, wt);
-    R b = pw_sum_blocked<DEPTH - 1, R, PL>(s + n2, n - n2, ld8, wt);
+    R a = pw_sum_blocked<DEPTH - 1, R, PL>(s, n2, ld, wt);
+    R b = pw_sum_blocked<DEPTH - 1, R, PL>(s + n2, n - n2, ld, wt);
     return a + b;
   }
 }
 
-// The row's reduceat value from ld(e) (one entry), ld8(e, x) (eight
-// consecutive entries, e.g. all their addresses formed before any load is
-// issued) and wt(e).
-template <class R, bool PL = false, class LD, class LD8, class WT>
-__device__ inline R reduceat_row_blocked8(int n, LD& ld, LD8& ld8, WT& wt) {
+template <class R, bool PL = false, class LD, class WT>
+__device__ inline R reduceat_row_blocked(int n, LD& ld, WT& wt) {
   if (n <= 0) return R(0);
   const R first = wt(0) * ld(0);
   if (n == 1) return first;
-  auto ld8_1 = [&](int e, R* x) { ld8(e + 1, x); };
+  auto ld1 = [&](int e) -> R { return ld(e + 1); };
   auto wt1 = [&](int e) -> R { return wt(e + 1); };
-  return first + pw_sum_blocked<3, R, PL>(0, n - 1, ld8_1, wt1);
-}
-
-template <class R, bool PL = false, class LD, class WT>
-__device__ inline R reduceat_row_blocked(int n, LD& ld, WT& wt) {
-  auto ld8 = [&](int e, R* x) {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) x[k] = ld(e + k);
-  };
-  return reduceat_row_blocked8<R, PL>(n, ld, ld8, wt);
+  return first + pw_sum_blocked<3, R, PL>(0, n - 1, ld1, wt1);
 }
 
 // Raw buffer loads: 32-bit byte offsets from a per-tile base; offsets past

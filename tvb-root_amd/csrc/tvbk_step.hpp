@@ -604,20 +604,8 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
             const EdgeRec r = getrec(e0 + e);
             return buf_load<RH>(rsrc, e < n ? slot_off(sm, r.delay) + r.coff + lane_off : 0xFFFFFFF0u);
           };
-          // a block's eight addresses are formed (eight record reads, one
-          // wait) before its eight loads are issued
-          auto ld8 = [&](int e, RH* x) {
-            uint32_t o[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-              const EdgeRec r = getrec(e0 + e + k);
-              o[k] = e + k < n ? slot_off(sm, r.delay) + r.coff + lane_off : 0xFFFFFFF0u;
-            }
-#pragma unroll
-            for (int k = 0; k < 8; ++k) x[k] = buf_load<RH>(rsrc, o[k]);
-          };
           auto wt = [&](int e) -> RH { return rec_weight<RH>(getrec(e0 + e)); };
-          acc = reduceat_row_blocked8<RH, true>(n, ld, ld8, wt);
+          acc = reduceat_row_blocked<RH, true>(n, ld, wt);
         }
         // the raw sum; the coupling's post runs in the state precision in
         // phase B (coupling.py:166-167: post of the float32 sum in float64)
@@ -882,7 +870,8 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
         if (u + 1 < nrow) ni = row_issue(RPc{}, w0 + (u + 1) * nwave, base_n, lane_off, vi, ei);
         const int ib = tid / T + u * (nthr / T);
         if (ib < N) {
-          // steps s and s+1 of the item (one copy of the update in the code)
+          // steps s and s+1 of the item (one copy of the update in the code:
+          // two copies compile six times slower for 1 % of speed)
 #pragma unroll 1
           for (int j = 0; j < (two ? 2 : 1); ++j)
             b_item(ib, cb + j * cstep, k + j, s + j, sir + j * TVBK_KMAX_MON, j ? ws1 : ws0, true, nullptr);

@@ -870,11 +870,8 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
         if (u + 1 < nrow) ni = row_issue(RPc{}, w0 + (u + 1) * nwave, base_n, lane_off, vi, ei);
         const int ib = tid / T + u * (nthr / T);
         if (ib < N) {
-          // steps s and s+1 of the item (one copy of the update in the code:
-          // two copies compile six times slower for 1 % of speed)
-#pragma unroll 1
-          for (int j = 0; j < (two ? 2 : 1); ++j)
-            b_item(ib, cb + j * cstep, k + j, s + j, sir + j * TVBK_KMAX_MON, j ? ws1 : ws0, true, nullptr);
+          b_item(ib, cb, k, s, sir, ws0, true, nullptr);
+          if (two) b_item(ib, cb + cstep, k + 1, s + 1, sir + TVBK_KMAX_MON, ws1, true, nullptr);
         }
         if (u < nrow) {
           const RH acc = row_finish(RPc{}, ef, nf, base_n, lane_off, vf);

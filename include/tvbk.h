@@ -399,8 +399,8 @@ int tvbk_plan_channels(tvbk_plan *plan, int32_t *n_channel);
  * first.  The pipelined loops need wave-uniform rows (one shared
  * connectome, n_channel * tile a multiple of 32), every delay >= 1 and no
  * stimulus or afferent monitor; the two-step one rows of exactly 32 values,
- * every delay >= 3, a model of up to two state variables whose parameters
- * every instance shares, and its four coupling buffers in LDS (environment
+ * every delay >= 3, a model of up to two state variables and its four
+ * coupling buffers in LDS (environment
  * TVBK_PIPE2=0 at plan creation keeps the one-step loop).  No reference
  * counterpart (a property of this backend's launch). */
 int tvbk_plan_loop(tvbk_plan *plan, int32_t *loop);

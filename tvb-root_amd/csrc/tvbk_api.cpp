@@ -671,17 +671,11 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   // gathers two steps of an edge), every delay >= 3, no afferent monitor,
   // history channels staged in the coupling buffers (as many as coupling
   // terms), models of up to two state variables (the loop holds two rows of
-  // loads beside the update: JansenRit's six variables spill) whose model
-  // parameters are plan constants (coupling and noise sweeps; the kernel
-  // reads them from its arguments); four coupling buffers and step records
-  // in LDS.  TVBK_PIPE2=0 keeps the one-step loop.
+  // loads beside the update: JansenRit's six variables spill); four
+  // coupling buffers and step records in LDS.  TVBK_PIPE2=0 keeps the
+  // one-step loop.
   p->pipe_k = 1;
-  bool model_const = d.node_param_mask == 0;
-  for (int b = 1; b < d.n_inst && model_const; ++b)
-    for (int k = 0; k < p->mi.nmp && model_const; ++k)
-      model_const = d.params[(size_t)b * d.n_param + k] == d.params[k];
-  if (p->uniform_rows && !p->batch_split && p->nch * T == 32 && p->nct == p->nch && !any_aff && p->mi.nsv <= 2 &&
-      model_const) {
+  if (p->uniform_rows && !p->batch_split && p->nch * T == 32 && p->nct == p->nch && !any_aff && p->mi.nsv <= 2) {
     int dmin2 = 1 << 30;
     for (const ConnDesc& cd : p->conns) dmin2 = std::min(dmin2, cd.nnz > 0 ? cd.d_min : 0);
     const char* envp = std::getenv("TVBK_PIPE2");

@@ -1182,12 +1182,11 @@ hipError_t launch_step_t(const StepArgs& a, int ntiles, int nthreads, size_t lds
   const uint64_t mmask = M::NDP >= 64 ? ~0ull : ((1ull << M::NDP) - 1);
   const bool um = pipe && (a.par_umask & mmask) == mmask && !a.spatial;
   // the two-step loop is built for models of up to two state variables
-  // whose model parameters are plan constants (the host plans it only for
-  // those)
+  // (the host plans it only for those)
   auto kern = pipe ? (um ? k_step<RH, R, M, CF, 1, true> : k_step<RH, R, M, CF, 1, false>)
                    : loop == TVBK_LOOP_TIME_BLOCKED ? k_step<RH, R, M, CF, 2, false> : k_step<RH, R, M, CF, 0, false>;
   if constexpr (M::NSV <= 2) {
-    if (loop == TVBK_LOOP_PIPELINED2 && um) kern = k_step<RH, R, M, CF, 3, true>;
+    if (loop == TVBK_LOOP_PIPELINED2) kern = um ? k_step<RH, R, M, CF, 3, true> : k_step<RH, R, M, CF, 3, false>;
   }
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -148,12 +148,15 @@ static const int NCH_OF_MODEL_CF[4][4] = {{1, 0, 1, 1}, {2, 0, 2, 1}, {2, 1, 2, 
 static void check_tile_common(const StepArgs& a, const TileDesc& td, const ConnDesc& cd, int nch, int hsize,
                               int esize, int nsv, int64_t n_call, int nmon_kind_bold[TVBK_KMAX_MON], const char* who) {
   const int T = a.T, N = cd.n_node, H = cd.horizon;
-  // ring: the corners of every coordinate range through ring_index
-  for (int slot : {0, H - 1})
+  // ring: the corners of every coordinate range through ring_index; the
+  // two-step pipelined loop also reads and writes slot H, the mirror of
+  // slot 0
+  const int hslots = a.pipe_k == 2 ? H + 1 : H;
+  for (int slot : {0, hslots - 1})
     for (int i : {td.node0, td.node1 - 1})
       for (int ch : {0, nch - 1})
         for (int t : {0, T - 1}) {
-          const int64_t e = td.ring_off + ring_index(T, H, N, nch, slot, i, ch, t);
+          const int64_t e = td.ring_off + ring_index(T, hslots, N, nch, slot, i, ch, t);
           need(a.ring, e, e, hsize, who);
         }
   need(a.img, td.img_off, td.img_off + a.img_bytes - 1, 1, "tile image");
@@ -190,6 +193,14 @@ hipError_t launch_step(int precision, int model, int cf, const StepArgs& a, int 
   const int offs[] = {a.lds_state, a.lds_c, a.lds_par};
   for (int o : offs)
     if (o < 0 || (size_t)o > lds) violation("LDS offset " + std::to_string(o) + " past " + std::to_string(lds));
+  if (a.pipe_k == 2) {
+    // two-step pipelined loop: four coupling buffers of the tile's nodes and
+    // four monitor step records per monitor slot, inside the dynamic LDS
+    if (!a.uniform_rows || nch * a.T != 32) violation("two-step loop without rows of 32 values");
+    const int64_t cb = (int64_t)a.lds_c + 4LL * a.lds_nodes * nch * a.T * hsize;
+    if ((size_t)cb > lds) violation("two-step coupling buffers past the LDS");
+    if ((size_t)a.lds_si + 4u * TVBK_KMAX_MON * TVBK_STEPINFO_BYTES > lds) violation("two-step step records past the LDS");
+  }
   int is_bold[TVBK_KMAX_MON] = {0};
   for (int m = 0; m < a.n_mon; ++m) {
     is_bold[m] = a.mon[m].kind == TVBK_MON_BOLD;

@@ -142,6 +142,7 @@ struct tvbk_plan {
   // pipelined loop's steps per pass (2: TVBK_LOOP_PIPELINED2, whose rings
   // carry one mirror slot past the horizon)
   int pipe_k = 1;
+  int row_shift = 0;   // two-step loop: rows fewer for the extra-pass waves
   int64_t img_bytes = 0;
   bool csr_in_lds = false;
   bool uniform_rows = false;   // batch-lane, n_channel * T a multiple of 32
@@ -1113,6 +1114,21 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   HIPCHK(p, hipEventCreate(&p->ev1));
   if (const char* env = std::getenv("TVBK_GRAPH"))
     if (*env && std::atoi(env) == 0) p->graph_off = true;
+  // two-step loop: the first E waves carry one more integration pass (two
+  // steps of an item, about four rows' time on the sweep: measured, rows
+  // shifted 0..4 -> 10.74, 10.65, 10.53, 10.39, 10.99 us/step); they take
+  // that many rows fewer.  TVBK_ROW_SHIFT=n sets the shift.
+  if (p->pipe_k == 2) {
+    const int N = p->conns[0].n_node, nw = p->waves, nthr = 64 * nw, T = p->T;
+    const int npass = (N + nthr / T - 1) / (nthr / T), npw = 64 / T;
+    const int E = (N - (npass - 1) * (nthr / T) + npw - 1) / npw;
+    if (E > 0 && E < nw) {
+      const double re = (double)N / nw - 4.0 * (nw - E) / nw;
+      p->row_shift = std::max(0, N / nw - std::max(0, (int)std::lround(re)));
+    }
+  }
+  if (const char* env = std::getenv("TVBK_ROW_SHIFT"))
+    if (*env) p->row_shift = std::max(0, std::atoi(env));
   HIPCHK(p, hipStreamSynchronize(p->stream));
   *out = p;
   return TVBK_OK;
@@ -1541,6 +1557,7 @@ int tvbk_run_steps(tvbk_plan* p, int64_t n) {
   a.lds_nodes = p->n_lds;
   a.no_pipe = p->batch_split ? 1 : 0;
   a.pipe_k = p->pipe_k;
+  a.row_shift = p->row_shift;
   a.lds_c = p->lds_c;
   a.lds_si = p->lds_si;
   a.tb_k = p->tb_k;

@@ -143,6 +143,9 @@ struct StepArgs {
   // n_channel * T = 32 values, every delay >= 3, rings of H + 1 slots whose
   // last mirrors slot 0)
   int32_t no_pipe, pipe_k;
+  // two-step loop: rows fewer than N / waves for the waves that carry an
+  // extra integration pass
+  int32_t row_shift, pad_rs;
 };
 
 // Node -> area reduction of one monitor's subsampled output (GlobalAverage,

@@ -1114,16 +1114,17 @@ static int plan_build(const tvbk_plan_desc* desc, void* stream, tvbk_plan** out,
   HIPCHK(p, hipEventCreate(&p->ev1));
   if (const char* env = std::getenv("TVBK_GRAPH"))
     if (*env && std::atoi(env) == 0) p->graph_off = true;
-  // two-step loop: the first E waves carry one more integration pass (two
-  // steps of an item, about four rows' time on the sweep: measured, rows
-  // shifted 0..4 -> 10.74, 10.65, 10.53, 10.39, 10.99 us/step); they take
-  // that many rows fewer.  TVBK_ROW_SHIFT=n sets the shift.
-  if (p->pipe_k == 2) {
+  // pipelined loops: the first E waves carry one more integration pass (one
+  // step of an item costs about two rows' time on the sweep; the two-step
+  // loop's pass, two steps: rows shifted 0..4 -> 10.74, 10.65, 10.53, 10.39,
+  // 10.99 us/step); they take that many rows fewer.  TVBK_ROW_SHIFT=n sets
+  // the shift.
+  if (p->uniform_rows && !p->batch_split) {
     const int N = p->conns[0].n_node, nw = p->waves, nthr = 64 * nw, T = p->T;
     const int npass = (N + nthr / T - 1) / (nthr / T), npw = 64 / T;
     const int E = (N - (npass - 1) * (nthr / T) + npw - 1) / npw;
     if (E > 0 && E < nw) {
-      const double re = (double)N / nw - 4.0 * (nw - E) / nw;
+      const double re = (double)N / nw - 2.0 * p->pipe_k * (nw - E) / nw;
       p->row_shift = std::max(0, N / nw - std::max(0, (int)std::lround(re)));
     }
   }

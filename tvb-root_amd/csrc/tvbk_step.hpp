@@ -775,14 +775,38 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
   // previous barrier, so each wave interleaves its rows of phase A(s+1) with
   // its updates of phase B(s) -- a row's loads are in flight while the wave
   // integrates -- and one barrier per step orders both against the next.
+  // rows of this wave in the pipelined loops: a contiguous range (the
+  // scalar records of a wave's rows share cache lines).  The first E waves
+  // carry one more integration pass than the others (when the passes do
+  // not divide the nodes); they take a.row_shift rows fewer than
+  // N / nwave, the others share the rest
+  auto row_range = [&](int npass, int& r0, int& nrow) {
+    const int npw = 64 / T;                              // nodes per wave and pass
+    const int E = (N - (npass - 1) * (nthr / T) + npw - 1) / npw;
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    if (E > 0 && E < nwave) {
+      const int re0 = N / nwave - a.row_shift, re = re0 > 0 ? re0 : 0;
+      const int rest = N - E * re, q = nwave - E, bq = rest / q, xq = rest - (rest / q) * q;
+      if (wv < E) {
+        r0 = wv * re;
+        nrow = re;
+      } else {
+        const int ix = wv - E;
+        nrow = bq + (ix < xq ? 1 : 0);
+        r0 = E * re + ix * bq + (ix < xq ? ix : xq);
+      }
+    } else {
+      const int bq = N / nwave, xq = N - (N / nwave) * nwave;
+      nrow = bq + (wv < xq ? 1 : 0);
+      r0 = wv * bq + (wv < xq ? wv : xq);
+    }
+  };
   if constexpr (PIPE == 1) {
     phase_a_uni(s_c, 1);                                // coupling of the first step
     __syncthreads();
-    // rows from the last wave down: the first waves carry the extra phase B
-    // pass when N * T is not a multiple of the block
-    const int w0 = __builtin_amdgcn_readfirstlane(nwave - 1 - wave);
-    const int nrow = (N - w0 + nwave - 1) / nwave;      // rows of this wave
     const int npass = (N + nthr / T - 1) / (nthr / T);  // phase B passes
+    int r0, nrow;
+    row_range(npass, r0, nrow);
     const int nu = nrow > npass ? nrow : npass;
     const uint32_t lane_off = lane < QN ? (uint32_t)lane * (uint32_t)sizeof(RH) : 0u;
     for (int k = 0; k < nsteps; ++k, ++s) {
@@ -798,12 +822,12 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
       // update u runs, then row u is reduced; va / vb alternate
       RH va[RP], vb[RP];
       int ea = 0, na = 0, eb = 0, nb = 0;
-      if (nrow > 0) na = row_issue(RPc{}, w0, base_n, lane_off, va, ea);
+      if (nrow > 0) na = row_issue(RPc{}, r0, base_n, lane_off, va, ea);
       auto body = [&](int u, RH* vf, int ef, int nf, RH* vi, int& ei, int& ni) {
-        if (u + 1 < nrow) ni = row_issue(RPc{}, w0 + (u + 1) * nwave, base_n, lane_off, vi, ei);
+        if (u + 1 < nrow) ni = row_issue(RPc{}, r0 + u + 1, base_n, lane_off, vi, ei);
         const int ib = tid / T + u * (nthr / T);
         if (ib < N) b_item(ib, cbuf, k, s, kc.si[buf], wslot, NCT == NCH, nullptr);
-        if (u < nrow) row_store(cnext, w0 + u * nwave, lane, row_finish(RPc{}, ef, nf, base_n, lane_off, vf));
+        if (u < nrow) row_store(cnext, r0 + u, lane, row_finish(RPc{}, ef, nf, base_n, lane_off, vf));
       };
       for (int u = 0; u < nu; u += 2) {
         body(u, va, ea, na, vb, eb, nb);
@@ -836,33 +860,9 @@ __global__ void __launch_bounds__(sizeof(RH) == 8 ? 512 : 1024) k_step(StepArgs 
       }
     phase_a_uni(s_c, 2);                                // coupling of steps 0 and 1
     __syncthreads();
-    // rows of this wave: a contiguous range.  The first E waves carry one
-    // more integration pass than the others (when the passes do not divide
-    // the nodes); they take a.row_shift rows fewer than N / nwave, the
-    // others share the rest
     const int npass = (N + nthr / T - 1) / (nthr / T);  // integration passes
-    const int npw = 64 / T;                              // nodes per wave and pass
-    const int E = (N - (npass - 1) * (nthr / T) + npw - 1) / npw;
     int r0, nrow;
-    {
-      const int wv = __builtin_amdgcn_readfirstlane(wave);
-      if (E > 0 && E < nwave) {
-        const int re0 = N / nwave - a.row_shift, re = re0 > 0 ? re0 : 0;
-        const int rest = N - E * re, q = nwave - E, bq = rest / q, xq = rest - (rest / q) * q;
-        if (wv < E) {
-          r0 = wv * re;
-          nrow = re;
-        } else {
-          const int ix = wv - E;
-          nrow = bq + (ix < xq ? 1 : 0);
-          r0 = E * re + ix * bq + (ix < xq ? ix : xq);
-        }
-      } else {
-        const int bq = N / nwave, xq = N - (N / nwave) * nwave;
-        nrow = bq + (wv < xq ? 1 : 0);
-        r0 = wv * bq + (wv < xq ? wv : xq);
-      }
-    }
+    row_range(npass, r0, nrow);
     const int nu = nrow > npass ? nrow : npass;
     const int hl = lane >> 5, ql = lane & 31;           // step half, value of the row
     const uint32_t lane_off = (uint32_t)ql * (uint32_t)sizeof(RH) + (uint32_t)hl * sstride;
